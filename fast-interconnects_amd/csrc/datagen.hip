@@ -1,0 +1,138 @@
+// Seeded, counter-based relation generators (device side).
+//
+// The reference generates relations with unseeded thread_rng
+// (datagen/src/relation.rs:76-290), so runs cannot be replayed.  These
+// generators are this library's own definition: every element is a pure
+// function of (seed, stream, index), so the GPU produces, in parallel, the
+// same integers as a sequential CPU implementation of the same formulas.
+//   mix64:      SplitMix64 finaliser
+//   rand_u64:   mix64(mix64(seed ^ mix64(stream + G)) + (i + 1) * G),
+//               G = 0x9e3779b97f4a7c15
+//   permutation: 4-round balanced Feistel network on 2*ceil(bits/2) bits
+//               with cycle walking (a bijection on [0, n))
+//   uniform:    lo + mulhi64(rand_u64, range)
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace fij {
+namespace {
+
+constexpr uint64_t kGolden = 0x9e3779b97f4a7c15ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t rand_u64(uint64_t seed, uint64_t stream, uint64_t i) {
+  const uint64_t k = mix64(seed ^ mix64(stream + kGolden));
+  return mix64(k + (i + 1) * kGolden);
+}
+
+__device__ uint64_t feistel_perm(uint64_t i, uint64_t n, uint64_t seed, uint32_t h, uint64_t mask,
+                                 const uint64_t* round_keys) {
+  uint64_t x = i;
+  do {
+    uint64_t l = x >> h, r = x & mask;
+#pragma unroll
+    for (int round = 0; round < 4; ++round) {
+      const uint64_t f = mix64(r ^ round_keys[round]) & mask;
+      const uint64_t nl = r;
+      r = l ^ f;
+      l = nl;
+    }
+    x = (l << h) | r;
+  } while (x >= n);
+  return x;
+}
+
+template <typename K>
+__global__ void gen_perm_kernel(K* out, uint64_t n, uint64_t seed, int64_t base) {
+  if (n <= 1) {
+    if (n == 1 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = static_cast<K>(base);
+    return;
+  }
+  const uint32_t b = 64u - __builtin_clzll(n - 1);
+  const uint32_t h = (b + 1u) / 2u;
+  const uint64_t mask = (1ull << h) - 1ull;
+  uint64_t rk[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rk[r] = mix64(seed + (r + 1) * kGolden);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = static_cast<K>(base + static_cast<int64_t>(feistel_perm(i, n, seed, h, mask, rk)));
+}
+
+template <typename K>
+__global__ void gen_uniform_kernel(K* out, uint64_t n, uint64_t seed, uint64_t stream_id, int64_t lo,
+                                   uint64_t range) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = static_cast<K>(lo + static_cast<int64_t>(__umul64hi(rand_u64(seed, stream_id, i), range)));
+}
+
+template <typename K>
+__global__ void gen_iota_kernel(K* out, uint64_t n, int64_t base) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = static_cast<K>(base + static_cast<int64_t>(i));
+}
+
+template <typename U>
+__global__ void fill_kernel(U* out, uint64_t n, U value) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = value;
+}
+
+uint32_t grid_for(uint64_t n) {
+  uint64_t g = (n + 255) / 256;
+  if (g > 16384) g = 16384;
+  return g ? static_cast<uint32_t>(g) : 1u;
+}
+
+}  // namespace
+
+hipError_t launch_gen_perm_keys(KeyType t, void* out, uint64_t n, uint64_t seed, int64_t base, hipStream_t stream) {
+  if (t == KeyType::I32)
+    hipLaunchKernelGGL(gen_perm_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream,
+                       static_cast<int32_t*>(out), n, seed, base);
+  else
+    hipLaunchKernelGGL(gen_perm_kernel<int64_t>, dim3(grid_for(n)), dim3(256), 0, stream,
+                       static_cast<int64_t*>(out), n, seed, base);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_uniform(KeyType t, void* out, uint64_t n, uint64_t seed, uint64_t stream_id, int64_t lo,
+                              uint64_t range, hipStream_t stream) {
+  if (t == KeyType::I32)
+    hipLaunchKernelGGL(gen_uniform_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream,
+                       static_cast<int32_t*>(out), n, seed, stream_id, lo, range);
+  else
+    hipLaunchKernelGGL(gen_uniform_kernel<int64_t>, dim3(grid_for(n)), dim3(256), 0, stream,
+                       static_cast<int64_t*>(out), n, seed, stream_id, lo, range);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_iota(KeyType t, void* out, uint64_t n, int64_t base, hipStream_t stream) {
+  if (t == KeyType::I32)
+    hipLaunchKernelGGL(gen_iota_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream,
+                       static_cast<int32_t*>(out), n, base);
+  else
+    hipLaunchKernelGGL(gen_iota_kernel<int64_t>, dim3(grid_for(n)), dim3(256), 0, stream,
+                       static_cast<int64_t*>(out), n, base);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint64_t>, dim3(grid_for(n)), dim3(256), 0, stream, out, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(grid_for(n)), dim3(256), 0, stream, out, n, value);
+  return hipGetLastError();
+}
+
+}  // namespace fij

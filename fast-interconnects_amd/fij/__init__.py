@@ -1,0 +1,694 @@
+"""Python mirror of the reference's sql-ops operator API over libfij.so.
+
+The classes follow sql-ops' Rust types (RadixBits, PartitionOffsets,
+PartitionedRelation, GpuRadixPartitioner, CudaRadixJoin, CudaHashJoin,
+HashTable) and call the C ABI declared in include/fij.h through ctypes.
+Device memory, copies and streams also go through the library (DeviceArray,
+Stream, Event), so a process holds exactly one HIP runtime.  If libfij.so is
+missing, importing this module raises, so no caller can silently fall back
+to another implementation.
+"""
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfij.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python fast-interconnects_amd/build.py`"
+    )
+lib = C.CDLL(LIB_PATH)
+
+
+# --------------------------------------------------------------------------
+# enums (include/fij.h)
+
+
+class Status(enum.IntEnum):
+    OK = 0
+    INVALID_ARGUMENT = 1
+    INTEGER_OVERFLOW = 2
+    DEVICE_ERROR = 3
+    RUNTIME_ERROR = 4
+    UNSUPPORTED = 5
+    OUT_OF_MEMORY = 6
+
+
+class KeyType(enum.IntEnum):
+    I32 = 0
+    I64 = 1
+
+
+class HistogramAlgorithm(enum.IntEnum):
+    Chunked = 0
+    Contiguous = 1
+
+
+class RadixPartitionAlgorithm(enum.IntEnum):
+    NC = 0
+    LASWWC = 1
+    SSWWCv2 = 2
+    SSWWCv2G = 3
+    HSSWWCv4 = 4
+
+
+class RadixPass(enum.IntEnum):
+    First = 0
+    Second = 1
+    Third = 2
+
+
+class HashingScheme(enum.IntEnum):
+    Perfect = 0
+    LinearProbing = 1
+    BucketChaining = 2
+
+
+class FijError(RuntimeError):
+    def __init__(self, status, message):
+        self.status = Status(status)
+        super().__init__(f"{self.status.name}: {message}")
+
+
+# --------------------------------------------------------------------------
+# C structs
+
+
+class _RadixBits(C.Structure):
+    _fields_ = [("pass_bits", C.c_int32 * 3)]
+
+
+class _PartitionOffsets(C.Structure):
+    _fields_ = [
+        ("offsets", C.c_void_p),
+        ("local_offsets", C.c_void_p),
+        ("data_len", C.c_uint64),
+        ("chunks", C.c_uint32),
+        ("max_chunks", C.c_uint32),
+        ("radix_bits", C.c_uint32),
+        ("type", C.c_uint32),
+    ]
+
+
+class _PartitionedRelation(C.Structure):
+    _fields_ = [
+        ("relation", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("relation_len", C.c_uint64),
+        ("len", C.c_uint64),
+        ("chunks", C.c_uint32),
+        ("radix_bits", C.c_uint32),
+        ("type", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class JoinAggregate(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("sum_build", C.c_uint64), ("sum_probe", C.c_uint64)]
+
+
+class _DeviceLimits(C.Structure):
+    _fields_ = [
+        ("compute_units", C.c_uint32),
+        ("wave_size", C.c_uint32),
+        ("lds_bytes_per_block", C.c_uint32),
+        ("max_threads_per_block", C.c_uint32),
+        ("global_mem_bytes", C.c_uint64),
+        ("arch_name", C.c_char * 64),
+    ]
+
+
+LEN_UNSET = 0xFFFFFFFFFFFFFFFF
+
+_u32, _u64, _i32, _i64, _vp = C.c_uint32, C.c_uint64, C.c_int32, C.c_int64, C.c_void_p
+
+
+def _proto(name, restype, *argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    return f
+
+
+_proto("fij_last_error", C.c_char_p)
+_proto("fij_version", C.c_char_p)
+_proto("fij_device_limits_get", C.c_int, C.c_int, C.POINTER(_DeviceLimits))
+_proto("fij_device_status", C.c_int, _vp, C.c_int, C.POINTER(_u32))
+_proto("fij_padding_len", _u32, C.c_int)
+_proto("fij_input_chunk_size", C.c_int, C.c_int, _u64, _u32, C.POINTER(_u64))
+_proto("fij_partition_offsets_layout", C.c_int, C.c_int, _u32, _u32, C.POINTER(_u32), C.POINTER(_u64), C.POINTER(_u64))
+_proto("fij_partitioned_relation_layout", C.c_int, C.c_int, _u64, C.c_int, _u32, _u32, C.POINTER(_u32), C.POINTER(_u64), C.POINTER(_u64))
+_proto("fij_partitioned_relation_padded_len", _u64, C.POINTER(_PartitionedRelation))
+_proto("fij_partitioned_relation_capacity", _u64, C.POINTER(_PartitionedRelation))
+_proto("fij_partitioned_relation_resize", C.c_int, C.POINTER(_PartitionedRelation), _u64)
+_proto("fij_partitioned_relation_partition_len", C.c_int, C.POINTER(_PartitionedRelation), _vp, _u32, C.POINTER(_u64))
+_proto("fij_partition_offsets_partition_len", C.c_int, C.POINTER(_PartitionOffsets), _vp, _u32, C.POINTER(_u64))
+_proto("fij_partitioner_create", C.c_int, C.c_int, C.c_int, _RadixBits, _u32, _u32, _u64, C.POINTER(_vp))
+_proto("fij_partitioner_destroy", C.c_int, _vp)
+_proto("fij_prefix_sum", C.c_int, _vp, C.c_int, C.c_int, _vp, _u64, C.POINTER(_PartitionOffsets), _vp)
+_proto("fij_prefix_sum_and_copy_with_payload", C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _u64, _vp, _vp, _u64, C.POINTER(_PartitionOffsets), _vp)
+_proto("fij_prefix_sum_and_transform", C.c_int, _vp, C.c_int, C.c_int, _u32, C.POINTER(_PartitionedRelation), _vp, _vp, _u64, C.POINTER(_PartitionOffsets), _vp)
+_proto("fij_preallocate_partition_state", C.c_int, _vp, C.c_int, C.c_int)
+_proto("fij_partition", C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _u64, C.POINTER(_PartitionOffsets), C.POINTER(_PartitionedRelation), _vp)
+_proto("fij_radix_join_create", C.c_int, C.c_int, _RadixBits, C.c_int, _u32, _u32, C.POINTER(_vp))
+_proto("fij_radix_join_destroy", C.c_int, _vp)
+_proto("fij_radix_join_join", C.c_int, _vp, C.c_int, C.POINTER(_PartitionedRelation), C.POINTER(_PartitionedRelation), _vp, _u64, _vp, _u64, _vp, _vp)
+_proto("fij_join_assign_tasks", C.c_int, C.POINTER(_PartitionedRelation), _u32, _u32, _vp, _vp)
+_proto("fij_np_table_init", C.c_int, C.c_int, _vp, _u64, _vp)
+_proto("fij_np_build", C.c_int, C.c_int, C.c_int, C.c_int, _vp, _u64, _vp, _vp, _u64, _u32, _u32, _vp)
+_proto("fij_np_probe_sum", C.c_int, C.c_int, C.c_int, _vp, _u64, _vp, _vp, _u64, _vp, _u64, _vp, _u32, _u32, _vp)
+_proto("fij_gen_permutation", C.c_int, C.c_int, _vp, _u64, _u64, _i64, _vp)
+_proto("fij_gen_uniform", C.c_int, C.c_int, _vp, _u64, _u64, _u64, _i64, _u64, _vp)
+_proto("fij_gen_iota", C.c_int, C.c_int, _vp, _u64, _i64, _vp)
+
+
+def check(status):
+    if status != 0:
+        raise FijError(status, lib.fij_last_error().decode())
+
+
+def version():
+    return lib.fij_version().decode()
+
+
+# --------------------------------------------------------------------------
+# runtime plumbing: device buffers and streams through the C ABI
+
+_proto("fij_device_count", C.c_int, C.POINTER(C.c_int))
+_proto("fij_mem_alloc", C.c_int, _u64, C.POINTER(_vp))
+_proto("fij_mem_free", C.c_int, _vp)
+_proto("fij_mem_alloc_host", C.c_int, _u64, C.POINTER(_vp))
+_proto("fij_mem_free_host", C.c_int, _vp)
+_proto("fij_memcpy", C.c_int, _vp, _vp, _u64, _vp)
+_proto("fij_memcpy_async", C.c_int, _vp, _vp, _u64, _vp)
+_proto("fij_mem_fill_u32", C.c_int, _vp, _u32, _u64, _vp)
+_proto("fij_mem_fill_u64", C.c_int, _vp, _u64, _u64, _vp)
+_proto("fij_stream_create", C.c_int, C.POINTER(_vp))
+_proto("fij_stream_destroy", C.c_int, _vp)
+_proto("fij_stream_synchronize", C.c_int, _vp)
+_proto("fij_device_synchronize", C.c_int)
+_proto("fij_event_create", C.c_int, C.POINTER(_vp))
+_proto("fij_event_destroy", C.c_int, _vp)
+_proto("fij_event_record", C.c_int, _vp, _vp)
+_proto("fij_event_elapsed_ms", C.c_int, _vp, _vp, C.POINTER(C.c_float))
+
+
+def device_count():
+    c = C.c_int(0)
+    check(lib.fij_device_count(C.byref(c)))
+    return c.value
+
+
+def synchronize():
+    check(lib.fij_device_synchronize())
+
+
+class DeviceArray:
+    """A typed device buffer (the role of rustacuda's DeviceBuffer)."""
+
+    def __init__(self, n, dtype, _ptr=None, _owner=None):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self._owner = _owner
+        if _ptr is not None:
+            self.ptr = _ptr
+        else:
+            p = _vp()
+            check(lib.fij_mem_alloc(max(self.n, 1) * self.dtype.itemsize, C.byref(p)))
+            self.ptr = p.value
+            self._owner = None
+
+    def __del__(self):
+        if self._owner is None and getattr(self, "ptr", None):
+            lib.fij_mem_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    @classmethod
+    def empty(cls, n, dtype):
+        return cls(n, dtype)
+
+    @classmethod
+    def zeros(cls, n, dtype):
+        a = cls(n, dtype)
+        a.fill(0)
+        return a
+
+    @classmethod
+    def from_host(cls, host, stream=None):
+        host = np.ascontiguousarray(host)
+        a = cls(host.size, host.dtype)
+        a.copy_from_host(host, stream)
+        return a
+
+    def numel(self):
+        return self.n
+
+    def __len__(self):
+        return self.n
+
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def data_ptr(self):
+        return self.ptr
+
+    def view(self, start, stop):
+        """Contiguous sub-range [start, stop) sharing this buffer."""
+        start, stop = int(start), int(stop)
+        assert 0 <= start <= stop <= self.n
+        return DeviceArray(stop - start, self.dtype, _ptr=self.ptr + start * self.dtype.itemsize,
+                           _owner=self if self._owner is None else self._owner)
+
+    def copy_from_host(self, host, stream=None):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.size <= self.n
+        check(lib.fij_memcpy(C.c_void_p(self.ptr), host.ctypes.data_as(C.c_void_p), host.nbytes, _stream(stream)))
+
+    def to_host(self, stream=None):
+        out = np.empty(self.n, self.dtype)
+        check(lib.fij_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes, _stream(stream)))
+        return out
+
+    def fill(self, value, stream=None):
+        if self.dtype.itemsize == 4:
+            v = int(np.array(value, self.dtype).view(np.uint32))
+            check(lib.fij_mem_fill_u32(C.c_void_p(self.ptr), v, self.n, _stream(stream)))
+        elif self.dtype.itemsize == 8:
+            v = int(np.array(value, self.dtype).view(np.uint64))
+            check(lib.fij_mem_fill_u64(C.c_void_p(self.ptr), v, self.n, _stream(stream)))
+        else:
+            raise ValueError("fill supports 4- and 8-byte elements")
+
+
+class Stream:
+    def __init__(self):
+        h = _vp()
+        check(lib.fij_stream_create(C.byref(h)))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.fij_stream_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def synchronize(self):
+        check(lib.fij_stream_synchronize(C.c_void_p(self.h)))
+
+
+class Event:
+    def __init__(self):
+        h = _vp()
+        check(lib.fij_event_create(C.byref(h)))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.fij_event_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def record(self, stream=None):
+        check(lib.fij_event_record(C.c_void_p(self.h), _stream(stream)))
+
+    def elapsed_ms(self, start):
+        ms = C.c_float()
+        check(lib.fij_event_elapsed_ms(C.c_void_p(start.h), C.c_void_p(self.h), C.byref(ms)))
+        return ms.value
+
+
+def _ktype(dtype):
+    if isinstance(dtype, KeyType):
+        return dtype
+    if dtype in ("i32",):
+        return KeyType.I32
+    if dtype in ("i64",):
+        return KeyType.I64
+    d = np.dtype(dtype)
+    if d == np.int32:
+        return KeyType.I32
+    if d == np.int64:
+        return KeyType.I64
+    raise ValueError(f"unsupported key type {dtype}")
+
+
+def numpy_dtype(ktype):
+    return np.int32 if ktype == KeyType.I32 else np.int64
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    return C.c_void_p(x.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.h)
+
+
+def device_limits(device=0):
+    out = _DeviceLimits()
+    check(lib.fij_device_limits_get(device, C.byref(out)))
+    return {
+        "compute_units": out.compute_units,
+        "wave_size": out.wave_size,
+        "lds_bytes_per_block": out.lds_bytes_per_block,
+        "max_threads_per_block": out.max_threads_per_block,
+        "global_mem_bytes": out.global_mem_bytes,
+        "arch_name": out.arch_name.decode(),
+    }
+
+
+def device_status(stream=None, clear=True):
+    v = _u32(0)
+    check(lib.fij_device_status(_stream(stream), int(clear), C.byref(v)))
+    return v.value
+
+
+def padding_len(ktype):
+    return lib.fij_padding_len(int(ktype))
+
+
+def input_chunk_size(ktype, data_len, num_chunks):
+    out = _u64(0)
+    check(lib.fij_input_chunk_size(int(ktype), data_len, num_chunks, C.byref(out)))
+    return out.value
+
+
+# --------------------------------------------------------------------------
+# RadixBits (sql-ops/src/partition.rs:62-120)
+
+
+class RadixBits:
+    def __init__(self, first=None, second=None, third=None):
+        self.pass_bits = [first, second, third]
+
+    def c(self):
+        s = _RadixBits()
+        for i, b in enumerate(self.pass_bits):
+            s.pass_bits[i] = -1 if b is None else int(b)
+        return s
+
+    def pass_radix_bits(self, p):
+        return self.pass_bits[int(p)]
+
+    def pass_fanout(self, p):
+        b = self.pass_bits[int(p)]
+        return None if b is None else 1 << b
+
+    def pass_ignore_bits(self, p):
+        return sum(b or 0 for b in self.pass_bits[: int(p)])
+
+    def radix_bits(self):
+        return sum(b or 0 for b in self.pass_bits)
+
+
+# --------------------------------------------------------------------------
+# PartitionOffsets / PartitionedRelation (partitioned_relation.rs)
+
+
+class PartitionOffsets:
+    """Device-resident partition offsets.  offsets: [chunks * F] u64;
+    local_offsets: [max_chunks * F] for the Contiguous algorithm."""
+
+    def __init__(self, histogram_algorithm, max_chunks, radix_bits, ktype):
+        self.ktype = _ktype(ktype)
+        chunks, olen, llen = _u32(), _u64(), _u64()
+        check(lib.fij_partition_offsets_layout(int(histogram_algorithm), max_chunks, radix_bits,
+                                               C.byref(chunks), C.byref(olen), C.byref(llen)))
+        self.offsets = DeviceArray.zeros(olen.value, np.uint64)
+        self.local_offsets = DeviceArray.zeros(llen.value, np.uint64) if llen.value else None
+        self.s = _PartitionOffsets()
+        self.s.offsets = self.offsets.data_ptr()
+        self.s.local_offsets = self.local_offsets.data_ptr() if self.local_offsets is not None else None
+        self.s.data_len = LEN_UNSET
+        self.s.chunks = chunks.value
+        self.s.max_chunks = max_chunks
+        self.s.radix_bits = radix_bits
+        self.s.type = int(self.ktype)
+
+    def len(self):
+        return None if self.s.data_len == LEN_UNSET else self.s.data_len
+
+    def num_chunks(self):
+        return self.s.chunks
+
+    def radix_bits(self):
+        return self.s.radix_bits
+
+    def fanout(self):
+        return 1 << self.s.radix_bits
+
+    def host_offsets(self):
+        return self.offsets.to_host()
+
+    def host_local_offsets(self):
+        return None if self.local_offsets is None else self.local_offsets.to_host()
+
+    def partition_len(self, pid, host_offsets=None):
+        h = self.host_offsets() if host_offsets is None else host_offsets
+        out = _u64()
+        check(lib.fij_partition_offsets_partition_len(C.byref(self.s), h.ctypes.data_as(C.c_void_p), pid, C.byref(out)))
+        return out.value
+
+
+class PartitionedRelation:
+    """Row-layout partitioned relation {key, value}[relation_len] in device memory."""
+
+    def __init__(self, length, histogram_algorithm, radix_bits, max_chunks, ktype, fill_null=False):
+        self.ktype = _ktype(ktype)
+        chunks, rlen, olen = _u32(), _u64(), _u64()
+        check(lib.fij_partitioned_relation_layout(int(self.ktype), length, int(histogram_algorithm), radix_bits,
+                                                  max_chunks, C.byref(chunks), C.byref(rlen), C.byref(olen)))
+        self.relation = DeviceArray(rlen.value * 2, numpy_dtype(self.ktype))
+        self.offsets = DeviceArray.zeros(olen.value, np.uint64)
+        self.s = _PartitionedRelation()
+        self.s.relation = self.relation.data_ptr()
+        self.s.offsets = self.offsets.data_ptr()
+        self.s.relation_len = rlen.value
+        self.s.len = length
+        self.s.chunks = chunks.value
+        self.s.radix_bits = radix_bits
+        self.s.type = int(self.ktype)
+        if fill_null:
+            self.fill_null()
+
+    def len(self):
+        return self.s.len
+
+    def padded_len(self):
+        return lib.fij_partitioned_relation_padded_len(C.byref(self.s))
+
+    def capacity(self):
+        return lib.fij_partitioned_relation_capacity(C.byref(self.s))
+
+    def resize(self, new_len):
+        check(lib.fij_partitioned_relation_resize(C.byref(self.s), new_len))
+
+    def num_chunks(self):
+        return self.s.chunks
+
+    def radix_bits(self):
+        return self.s.radix_bits
+
+    def fanout(self):
+        return 1 << self.s.radix_bits
+
+    def padding_len(self):
+        return padding_len(self.ktype)
+
+    def host_offsets(self):
+        return self.offsets.to_host()
+
+    def host_relation(self):
+        """(keys, values) numpy views of the whole relation, padding included."""
+        a = self.relation.to_host()
+        return a[0::2], a[1::2]
+
+    def fill_null(self, stream=None):
+        """Sets every tuple to {null key, 0} (as the reference's tests do)."""
+        check(lib.fij_np_table_init(int(self.ktype), _ptr(self.relation), self.s.relation_len, _stream(stream)))
+
+    def partition_len(self, pid, host_offsets=None):
+        h = self.host_offsets() if host_offsets is None else host_offsets
+        out = _u64()
+        check(lib.fij_partitioned_relation_partition_len(C.byref(self.s), h.ctypes.data_as(C.c_void_p), pid, C.byref(out)))
+        return out.value
+
+    def partition_range(self, chunk, pid, host_offsets=None):
+        """Index<(chunk, partition)> (partitioned_relation.rs:523-545) as a [begin, end) range."""
+        h = self.host_offsets() if host_offsets is None else host_offsets
+        F = self.fanout()
+        ofi = chunk * F + pid
+        begin = int(h[ofi])
+        end = int(h[ofi + 1]) - self.padding_len() if ofi + 1 < len(h) else self.padded_len()
+        return begin, end
+
+
+# --------------------------------------------------------------------------
+# GpuRadixPartitioner (gpu_radix_partition.rs:314-555)
+
+
+class GpuRadixPartitioner:
+    def __init__(self, histogram_algorithm, partition_algorithm, radix_bits, grid_size, block_size,
+                 dmem_buffer_bytes=8 * 1024):
+        if not isinstance(radix_bits, RadixBits):
+            radix_bits = RadixBits(radix_bits)
+        self.radix_bits = radix_bits
+        self.histogram_algorithm = HistogramAlgorithm(histogram_algorithm)
+        self.partition_algorithm = RadixPartitionAlgorithm(partition_algorithm)
+        self.grid_size = grid_size
+        self.block_size = block_size
+        h = _vp()
+        check(lib.fij_partitioner_create(int(histogram_algorithm), int(partition_algorithm), radix_bits.c(),
+                                         grid_size, block_size, dmem_buffer_bytes, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.fij_partitioner_destroy(self.h)
+            self.h = None
+
+    def prefix_sum(self, radix_pass, partition_attr, partition_offsets, stream=None):
+        kt = _ktype(partition_attr.dtype)
+        check(lib.fij_prefix_sum(self.h, int(radix_pass), int(kt), _ptr(partition_attr), partition_attr.numel(),
+                                 C.byref(partition_offsets.s), _stream(stream)))
+
+    def prefix_sum_and_copy_with_payload(self, radix_pass, src_key, src_pay, dst_key, dst_pay, partition_offsets,
+                                         stream=None):
+        kt = _ktype(src_key.dtype)
+        check(lib.fij_prefix_sum_and_copy_with_payload(
+            self.h, int(radix_pass), int(kt), _ptr(src_key), _ptr(src_pay), src_key.numel(), _ptr(dst_key),
+            _ptr(dst_pay), dst_key.numel(), C.byref(partition_offsets.s), _stream(stream)))
+
+    def prefix_sum_and_transform(self, radix_pass, partition_id, src_relation, dst_key, dst_pay, partition_offsets,
+                                 stream=None):
+        kt = _ktype(dst_key.dtype)
+        check(lib.fij_prefix_sum_and_transform(
+            self.h, int(radix_pass), int(kt), partition_id, C.byref(src_relation.s), _ptr(dst_key), _ptr(dst_pay),
+            dst_key.numel(), C.byref(partition_offsets.s), _stream(stream)))
+
+    def preallocate_partition_state(self, radix_pass, ktype):
+        check(lib.fij_preallocate_partition_state(self.h, int(radix_pass), int(_ktype(ktype))))
+
+    def partition(self, radix_pass, partition_attr, payload_attr, partition_offsets, partitioned_relation,
+                  stream=None):
+        if partition_attr.numel() != payload_attr.numel():
+            raise FijError(Status.INVALID_ARGUMENT, "Partition and payload attributes have different sizes")
+        kt = _ktype(partition_attr.dtype)
+        check(lib.fij_partition(self.h, int(radix_pass), int(kt), _ptr(partition_attr), _ptr(payload_attr),
+                                partition_attr.numel(), C.byref(partition_offsets.s),
+                                C.byref(partitioned_relation.s), _stream(stream)))
+        # The C call swapped the offset pointers; swap the owning buffers too.
+        partition_offsets.offsets, partitioned_relation.offsets = (
+            partitioned_relation.offsets, partition_offsets.offsets)
+        assert partitioned_relation.s.offsets == partitioned_relation.offsets.data_ptr()
+
+
+# --------------------------------------------------------------------------
+# CudaRadixJoin (sql-ops/src/join/cuda_radix_join.rs:105-164)
+
+
+class HipRadixJoin:
+    def __init__(self, radix_pass, radix_bits, hashing_scheme, grid_size, block_size):
+        if not isinstance(radix_bits, RadixBits):
+            radix_bits = RadixBits(radix_bits)
+        self.grid_size = grid_size
+        self.block_size = block_size
+        h = _vp()
+        check(lib.fij_radix_join_create(int(radix_pass), radix_bits.c(), int(hashing_scheme), grid_size, block_size,
+                                        C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.fij_radix_join_destroy(self.h)
+            self.h = None
+
+    def join(self, build_rel, probe_rel, result_set, task_assignments, aggregates=None, stream=None):
+        check(lib.fij_radix_join_join(
+            self.h, int(build_rel.ktype), C.byref(build_rel.s), C.byref(probe_rel.s), _ptr(result_set),
+            result_set.numel() if result_set is not None else 0, _ptr(task_assignments), task_assignments.numel(),
+            _ptr(aggregates), _stream(stream)))
+
+
+CudaRadixJoin = HipRadixJoin
+
+
+def assign_tasks(probe_rel, radix_bits, grid_size, task_assignments, stream=None):
+    check(lib.fij_join_assign_tasks(C.byref(probe_rel.s), radix_bits, grid_size, _ptr(task_assignments),
+                                    _stream(stream)))
+
+
+def aggregates_array(n):
+    """Zeroed device array of n JoinAggregate records (3 x u64 each)."""
+    return DeviceArray.zeros(3 * n, np.uint64)
+
+
+def sum_aggregates(agg):
+    a = agg.to_host().view(np.uint64).reshape(-1, 3)
+    with np.errstate(over="ignore"):
+        s = a.sum(axis=0, dtype=np.uint64)
+    return int(s[0]), int(s[1]), int(s[2])
+
+
+# --------------------------------------------------------------------------
+# No-partitioning join (sql-ops/src/join/no_partitioning_join.rs)
+
+
+class HashTable:
+    """HashTable::new_on_gpu: `entries` {key, value} slots set to the null key."""
+
+    def __init__(self, entries, ktype, stream=None):
+        self.ktype = _ktype(ktype)
+        self.entries = entries
+        self.mem = DeviceArray(2 * entries, numpy_dtype(self.ktype))
+        check(lib.fij_np_table_init(int(self.ktype), _ptr(self.mem), entries, _stream(stream)))
+
+
+class HipHashJoin:
+    def __init__(self, hashing_scheme, hash_table, build_dim=(1, 1024), probe_dim=(1, 1024), is_selective=False):
+        self.scheme = HashingScheme(hashing_scheme)
+        self.ht = hash_table
+        self.build_dim = build_dim
+        self.probe_dim = probe_dim
+        self.is_selective = is_selective
+
+    def build(self, keys, pays, stream=None):
+        if keys.numel() != pays.numel():
+            raise FijError(Status.INVALID_ARGUMENT, "Join and payload attributes have different sizes")
+        check(lib.fij_np_build(int(self.ht.ktype), int(self.scheme), int(self.is_selective), _ptr(self.ht.mem),
+                               self.ht.entries, _ptr(keys), _ptr(pays), keys.numel(), self.build_dim[0],
+                               self.build_dim[1], _stream(stream)))
+
+    def probe_sum(self, keys, pays, result_set=None, aggregates=None, stream=None):
+        check(lib.fij_np_probe_sum(int(self.ht.ktype), int(self.scheme), _ptr(self.ht.mem), self.ht.entries,
+                                   _ptr(keys), _ptr(pays), keys.numel(), _ptr(result_set),
+                                   result_set.numel() if result_set is not None else 0, _ptr(aggregates),
+                                   self.probe_dim[0], self.probe_dim[1], _stream(stream)))
+
+
+CudaHashJoin = HipHashJoin
+
+
+# --------------------------------------------------------------------------
+# data generation
+
+
+def gen_permutation(out, seed, base=1, stream=None):
+    check(lib.fij_gen_permutation(int(_ktype(out.dtype)), _ptr(out), out.numel(), seed, base, _stream(stream)))
+
+
+def gen_uniform(out, seed, stream_id, lo, count, stream=None):
+    check(lib.fij_gen_uniform(int(_ktype(out.dtype)), _ptr(out), out.numel(), seed, stream_id, lo, count,
+                              _stream(stream)))
+
+
+def gen_iota(out, base=0, stream=None):
+    check(lib.fij_gen_iota(int(_ktype(out.dtype)), _ptr(out), out.numel(), base, _stream(stream)))

@@ -1,0 +1,296 @@
+/*
+ * fij.h -- C ABI of the MI355X (gfx950) radix-join library.
+ *
+ * This is the drop-in boundary for the reference's sql-ops GPU partitioner
+ * and join operators (TU-Berlin-DIMA/fast-interconnects, sql-ops crate).
+ * In the reference, the Rust operator structs launch CUDA kernels by name
+ * from a fatbin (sql-ops/src/lib.rs:117-142).  Here every operator method is
+ * one extern "C" function taking plain pointers, lengths and a hipStream_t
+ * (passed as void*; NULL = default stream).  Functions are asynchronous on
+ * that stream unless stated otherwise and return an fij_status; the message
+ * of the last error on the calling thread is available from
+ * fij_last_error().
+ *
+ * Layouts are the reference's: tuples are {key, value} rows of 8 (i32) or 16
+ * (i64) bytes; every partition has 128 bytes of padding in front of it;
+ * offsets are u64 tuple indices.  Device buffers are owned by the caller
+ * except the partitioner's and join's internal state.
+ */
+#ifndef FIJ_H
+#define FIJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* sql-ops/src/error.rs:17-29 (ErrorKind). */
+typedef enum fij_status {
+  FIJ_OK = 0,
+  FIJ_INVALID_ARGUMENT = 1,
+  FIJ_INTEGER_OVERFLOW = 2,
+  FIJ_DEVICE_ERROR = 3,   /* ErrorKind::CudaError */
+  FIJ_RUNTIME_ERROR = 4,
+  FIJ_UNSUPPORTED = 5,    /* combinations the reference leaves unimplemented!() */
+  FIJ_OUT_OF_MEMORY = 6
+} fij_status;
+
+/* Key/payload element type: Tuple<i32, i32> or Tuple<i64, i64>. */
+typedef enum fij_type { FIJ_TYPE_I32 = 0, FIJ_TYPE_I64 = 1 } fij_type;
+
+/* sql-ops/src/partition/gpu_radix_partition.rs:194-219 (GpuHistogramAlgorithm). */
+typedef enum fij_histogram_algorithm {
+  FIJ_HISTOGRAM_CHUNKED = 0,
+  FIJ_HISTOGRAM_CONTIGUOUS = 1
+} fij_histogram_algorithm;
+
+/* sql-ops/src/partition/gpu_radix_partition.rs:241-293 (GpuRadixPartitionAlgorithm).
+ * NC is the direct scatter; the four write-combining variants all select
+ * this library's LDS line-buffered scatter (falling back to the direct
+ * scatter when the fanout's buffers exceed LDS). */
+typedef enum fij_partition_algorithm {
+  FIJ_PARTITION_NC = 0,
+  FIJ_PARTITION_LASWWC = 1,
+  FIJ_PARTITION_SSWWC_V2 = 2,
+  FIJ_PARTITION_SSWWC_V2G = 3,
+  FIJ_PARTITION_HSSWWC_V4 = 4
+} fij_partition_algorithm;
+
+/* sql-ops/src/partition.rs:55-60 (RadixPass). */
+typedef enum fij_radix_pass { FIJ_PASS_FIRST = 0, FIJ_PASS_SECOND = 1, FIJ_PASS_THIRD = 2 } fij_radix_pass;
+
+/* sql-ops/src/join/hashing_scheme.rs:19-43 (HashingScheme). */
+typedef enum fij_hashing_scheme {
+  FIJ_HASH_PERFECT = 0,
+  FIJ_HASH_LINEAR_PROBING = 1,
+  FIJ_HASH_BUCKET_CHAINING = 2
+} fij_hashing_scheme;
+
+/* sql-ops/src/partition.rs:62-120 (RadixBits): bits of each pass, a
+ * negative value means the pass is not specified (None). */
+typedef struct fij_radix_bits {
+  int32_t pass_bits[3];
+} fij_radix_bits;
+
+#define FIJ_LEN_UNSET UINT64_MAX
+
+/* sql-ops/src/partition/partitioned_relation.rs:63-70 (PartitionOffsets).
+ * chunks = max_chunks for Chunked and 1 for Contiguous; local_offsets holds
+ * max_chunks * fanout entries for Contiguous and is NULL for Chunked. */
+typedef struct fij_partition_offsets {
+  uint64_t *offsets;
+  uint64_t *local_offsets;
+  uint64_t data_len; /* FIJ_LEN_UNSET until a prefix sum ran */
+  uint32_t chunks;
+  uint32_t max_chunks;
+  uint32_t radix_bits;
+  uint32_t type; /* fij_type */
+} fij_partition_offsets;
+
+/* sql-ops/src/partition/partitioned_relation.rs:316-322 (PartitionedRelation). */
+typedef struct fij_partitioned_relation {
+  void *relation;        /* Tuple[relation_len] */
+  uint64_t *offsets;     /* [chunks * fanout] */
+  uint64_t relation_len; /* allocated tuples, padding included */
+  uint64_t len;          /* tuples, padding excluded */
+  uint32_t chunks;
+  uint32_t radix_bits;
+  uint32_t type;         /* fij_type */
+  uint32_t reserved;
+} fij_partitioned_relation;
+
+/* Join aggregate extension: the reference reports only SUM(probe.value)
+ * (sql-ops/src/join/cuda_radix_join.rs:17-30); this adds the match count and
+ * SUM(build.value).  All sums are wrapping 64-bit. */
+typedef struct fij_join_aggregate {
+  uint64_t count;
+  uint64_t sum_build;
+  uint64_t sum_probe;
+} fij_join_aggregate;
+
+typedef struct fij_device_limits {
+  uint32_t compute_units;
+  uint32_t wave_size;
+  uint32_t lds_bytes_per_block; /* usable LDS per block (opt-in maximum) */
+  uint32_t max_threads_per_block;
+  uint64_t global_mem_bytes;
+  char arch_name[64];
+} fij_device_limits;
+
+/* ----------------------------------------------------------- library ---- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *fij_last_error(void);
+/* Library version string. */
+const char *fij_version(void);
+/* Device properties used for launch decisions (the reference reads
+ * MultiprocessorCount / MaxSharedMemoryPerBlockOptin / WarpSize:
+ * gpu_radix_partition.rs:570-571,620-621,981). Synchronous. */
+fij_status fij_device_limits_get(int device, fij_device_limits *out);
+/* Reads and optionally clears the current device's kernel status word
+ * (bits: 1 transform length mismatch, 2 hash table overflow).  The
+ * reference traps with device asserts instead (radix_join.cu:131-132,
+ * 209-217).  Synchronous on `stream`. */
+fij_status fij_device_status(void *stream, int clear, uint32_t *out);
+
+/* ------------------------------------------------- runtime plumbing ---- */
+/* HIP-backed stand-ins for the boundary types the reference's host code takes
+ * from rustacuda and numa-gpu: DeviceBuffer / LockedBuffer and the Mem enum
+ * (numa-gpu/src/runtime/memory.rs:36-95), Stream and Event.  Copies with
+ * fij_memcpy complete before it returns; the fills are kernels (no memset
+ * nodes), asynchronous on `stream`. */
+fij_status fij_device_count(int *count);
+fij_status fij_mem_alloc(uint64_t bytes, void **out);
+fij_status fij_mem_free(void *ptr);
+fij_status fij_mem_alloc_host(uint64_t bytes, void **out);
+fij_status fij_mem_free_host(void *ptr);
+fij_status fij_memcpy(void *dst, const void *src, uint64_t bytes, void *stream);
+fij_status fij_memcpy_async(void *dst, const void *src, uint64_t bytes, void *stream);
+fij_status fij_mem_fill_u32(void *ptr, uint32_t value, uint64_t count, void *stream);
+fij_status fij_mem_fill_u64(void *ptr, uint64_t value, uint64_t count, void *stream);
+fij_status fij_stream_create(void **out);
+fij_status fij_stream_destroy(void *stream);
+fij_status fij_stream_synchronize(void *stream);
+fij_status fij_device_synchronize(void);
+fij_status fij_event_create(void **out);
+fij_status fij_event_destroy(void *event);
+fij_status fij_event_record(void *event, void *stream);
+/* Milliseconds between two recorded events; waits for `stop`. */
+fij_status fij_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* --------------------------------------------- layout helpers (host) ---- */
+
+/* partitioned_relation.rs:26-29: padding tuples per partition. */
+uint32_t fij_padding_len(fij_type type);
+/* partition_input_chunk.rs:24-35: canonical chunk length; INTEGER_OVERFLOW
+ * when a chunk reaches u32::MAX tuples. */
+fij_status fij_input_chunk_size(fij_type type, uint64_t data_len, uint32_t num_chunks, uint64_t *chunk_len);
+/* PartitionOffsets::new sizes (partitioned_relation.rs:74-103). */
+fij_status fij_partition_offsets_layout(fij_histogram_algorithm algo, uint32_t max_chunks, uint32_t radix_bits,
+                                        uint32_t *chunks, uint64_t *offsets_len, uint64_t *local_offsets_len);
+/* PartitionedRelation::new sizes (partitioned_relation.rs:335-362). */
+fij_status fij_partitioned_relation_layout(fij_type type, uint64_t len, fij_histogram_algorithm algo,
+                                           uint32_t radix_bits, uint32_t max_chunks, uint32_t *chunks,
+                                           uint64_t *relation_len, uint64_t *offsets_len);
+/* PartitionedRelation::padded_len / capacity / resize
+ * (partitioned_relation.rs:365-420). */
+uint64_t fij_partitioned_relation_padded_len(const fij_partitioned_relation *rel);
+uint64_t fij_partitioned_relation_capacity(const fij_partitioned_relation *rel);
+fij_status fij_partitioned_relation_resize(fij_partitioned_relation *rel, uint64_t new_len);
+/* partition_len of PartitionedRelation / PartitionOffsets
+ * (partitioned_relation.rs:429-459, :162-195).  `host_offsets` must be a
+ * host-readable copy of the offsets. */
+fij_status fij_partitioned_relation_partition_len(const fij_partitioned_relation *rel,
+                                                  const uint64_t *host_offsets, uint32_t partition_id,
+                                                  uint64_t *len);
+fij_status fij_partition_offsets_partition_len(const fij_partition_offsets *offs, const uint64_t *host_offsets,
+                                               uint32_t partition_id, uint64_t *len);
+
+/* ------------------------------------------------------ partitioner ---- */
+
+typedef struct fij_partitioner fij_partitioner;
+
+/* GpuRadixPartitioner::new (gpu_radix_partition.rs:329-383).  block is
+ * clamped to 1024 threads (the reference's default warp_size * 32 is 2048
+ * with 64-wide waves). */
+fij_status fij_partitioner_create(fij_histogram_algorithm histogram_algorithm,
+                                  fij_partition_algorithm partition_algorithm, fij_radix_bits radix_bits,
+                                  uint32_t grid_size, uint32_t block_size, uint64_t dmem_buffer_bytes,
+                                  fij_partitioner **out);
+fij_status fij_partitioner_destroy(fij_partitioner *p);
+
+/* GpuRadixPartitioner::prefix_sum (gpu_radix_partition.rs:398-406, impl
+ * :561-696; kernels radix_partition.cu:40-199).  Sets offsets->data_len. */
+fij_status fij_prefix_sum(fij_partitioner *p, fij_radix_pass pass, fij_type type, const void *partition_attr,
+                          uint64_t len, fij_partition_offsets *offsets, void *stream);
+
+/* GpuRadixPartitioner::prefix_sum_and_copy_with_payload
+ * (gpu_radix_partition.rs:431-451, impl :698-824; radix_partition.cu:207-315). */
+fij_status fij_prefix_sum_and_copy_with_payload(fij_partitioner *p, fij_radix_pass pass, fij_type type,
+                                                const void *src_partition_attr, const void *src_payload_attr,
+                                                uint64_t len, void *dst_partition_attr,
+                                                void *dst_payload_attr, uint64_t dst_len,
+                                                fij_partition_offsets *offsets, void *stream);
+
+/* GpuRadixPartitioner::prefix_sum_and_transform (gpu_radix_partition.rs:480-500,
+ * impl :826-957; radix_partition.cu:323-467).  dst_len is the length of the
+ * dst slices, i.e. the partition's length. */
+fij_status fij_prefix_sum_and_transform(fij_partitioner *p, fij_radix_pass pass, fij_type type,
+                                        uint32_t partition_id, const fij_partitioned_relation *src_relation,
+                                        void *dst_partition_attr, void *dst_payload_attr, uint64_t dst_len,
+                                        fij_partition_offsets *offsets, void *stream);
+
+/* GpuRadixPartitioner::preallocate_partition_state (gpu_radix_partition.rs:511-526). */
+fij_status fij_preallocate_partition_state(fij_partitioner *p, fij_radix_pass pass, fij_type type);
+
+/* GpuRadixPartitioner::partition (gpu_radix_partition.rs:535-554, impl
+ * :1022-1239; kernels radix_partition.cu:472-1477).  Afterwards the
+ * relation owns the offsets: relation->offsets and offsets->offsets are
+ * swapped (:1117-1121). */
+fij_status fij_partition(fij_partitioner *p, fij_radix_pass pass, fij_type type, const void *partition_attr,
+                         const void *payload_attr, uint64_t len, fij_partition_offsets *offsets,
+                         fij_partitioned_relation *relation, void *stream);
+
+/* --------------------------------------------------------- radix join ---- */
+
+typedef struct fij_radix_join fij_radix_join;
+
+/* CudaRadixJoin::new (sql-ops/src/join/cuda_radix_join.rs:116-130). */
+fij_status fij_radix_join_create(fij_radix_pass pass, fij_radix_bits radix_bits, fij_hashing_scheme scheme,
+                                 uint32_t grid_size, uint32_t block_size, fij_radix_join **out);
+fij_status fij_radix_join_destroy(fij_radix_join *j);
+
+/* CudaRadixJoin::join (cuda_radix_join.rs:133-164, impl :172-276; kernels
+ * radix_join.cu:58-311).  result_set: grid*block per-thread SUM(probe.value)
+ * partials, added to (caller zeroes), may be NULL; aggregates: grid
+ * per-block {count, sum_build, sum_probe}, added to, may be NULL;
+ * task_assignments: grid + 1 entries of device scratch. */
+fij_status fij_radix_join_join(fij_radix_join *j, fij_type type, const fij_partitioned_relation *build_rel,
+                               const fij_partitioned_relation *probe_rel, int64_t *result_set,
+                               uint64_t result_set_len, uint32_t *task_assignments,
+                               uint64_t task_assignments_len, fij_join_aggregate *aggregates, void *stream);
+
+/* gpu_radix_join_assign_tasks alone (radix_join.cu:58-98). */
+fij_status fij_join_assign_tasks(const fij_partitioned_relation *probe_rel, uint32_t radix_bits,
+                                 uint32_t grid_size, uint32_t *task_assignments, void *stream);
+
+/* ---------------------------------------------- no-partitioning join ---- */
+
+/* HashTable::new_on_gpu (sql-ops/src/join/no_partitioning_join.rs:588-634):
+ * sets every entry to {null key, 0}. */
+fij_status fij_np_table_init(fij_type type, void *table, uint64_t entries, void *stream);
+/* CudaHashJoin::build (no_partitioning_join.rs:255-268, impl :313-375; kernels
+ * no_partitioning_join.cu:115-132, :251-319).  is_selective skips null
+ * build keys (perfect hashing only). */
+fij_status fij_np_build(fij_type type, fij_hashing_scheme scheme, int is_selective, void *table, uint64_t entries,
+                        const void *keys, const void *payloads, uint64_t len, uint32_t grid_size,
+                        uint32_t block_size, void *stream);
+/* CudaHashJoin::probe_sum (no_partitioning_join.rs:270-284, impl :377-440; kernels
+ * no_partitioning_join.cu:225-249, :321-379).  result_set: grid*block u64
+ * partials of SUM(probe.value), added to; aggregates: grid entries, added
+ * to; either may be NULL. */
+fij_status fij_np_probe_sum(fij_type type, fij_hashing_scheme scheme, const void *table, uint64_t entries,
+                            const void *keys, const void *payloads, uint64_t len, uint64_t *result_set,
+                            uint64_t result_set_len, fij_join_aggregate *aggregates, uint32_t grid_size,
+                            uint32_t block_size, void *stream);
+
+/* ---------------------------------------------------- data generation ---- */
+/* Seeded replacements for the reference's unseeded generators
+ * (datagen/src/relation.rs:76-290); each element is a pure function of
+ * (seed, stream, index). */
+
+/* out[i] = base + perm(i), perm a seeded bijection on [0, n) (primary keys). */
+fij_status fij_gen_permutation(fij_type type, void *out, uint64_t n, uint64_t seed, int64_t base, void *stream);
+/* out[i] = lo + floor(u_i * range), u_i uniform (foreign keys, attributes). */
+fij_status fij_gen_uniform(fij_type type, void *out, uint64_t n, uint64_t seed, uint64_t stream_id, int64_t lo,
+                           uint64_t range, void *stream);
+/* out[i] = base + i (payloads). */
+fij_status fij_gen_iota(fij_type type, void *out, uint64_t n, int64_t base, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIJ_H */

@@ -1,0 +1,282 @@
+"""C ABI host-side behaviour, no GPU needed.
+
+* libfij.so loads and exports every function include/fij.h declares;
+* layout helpers follow partitioned_relation.rs / partition_input_chunk.rs;
+* argument validation returns the reference's error kinds
+  (sql-ops/src/error.rs:17-29) before anything touches a device, as the
+  reference's Rust checks do (gpu_radix_partition.rs:571-611, :1032-1080,
+  cuda_radix_join.rs:183-199).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fij
+from fij import FijError, Status
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FAKE = 1 << 40  # never dereferenced: every call below fails validation first
+
+
+def declared_functions():
+    text = open(os.path.join(REPO, "include", "fij.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fij_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(fij.lib, n), f"libfij.so does not export {n}"
+
+
+def test_version():
+    assert "gfx950" in fij.version()
+
+
+def test_padding_len():
+    # partitioned_relation.rs:26-29: PADDING_BYTES / sizeof(Tuple)
+    assert fij.padding_len(fij.KeyType.I32) == 16
+    assert fij.padding_len(fij.KeyType.I64) == 8
+
+
+@pytest.mark.parametrize("kt,n,chunks,expect", [
+    (fij.KeyType.I32, 1000, 3, 320),          # ceil(1000/3)=334, & ~31
+    (fij.KeyType.I64, 10 ** 6, 10, 100000),   # multiple of 16 already
+    (fij.KeyType.I64, 100, 4, 16),            # 25 & ~15
+    (fij.KeyType.I32, 100, 4, 0),             # 25 & ~31
+    (fij.KeyType.I64, 0, 7, 0),
+])
+def test_input_chunk_size(kt, n, chunks, expect):
+    assert fij.input_chunk_size(kt, n, chunks) == expect
+
+
+def test_input_chunk_size_overflow():
+    with pytest.raises(FijError) as e:
+        fij.input_chunk_size(fij.KeyType.I64, 1 << 33, 1)
+    assert e.value.status == Status.INTEGER_OVERFLOW
+
+
+def test_layouts():
+    chunks, olen, llen = C.c_uint32(), C.c_uint64(), C.c_uint64()
+    fij.check(fij.lib.fij_partition_offsets_layout(0, 10, 9, C.byref(chunks), C.byref(olen), C.byref(llen)))
+    assert (chunks.value, olen.value, llen.value) == (10, 5120, 0)
+    fij.check(fij.lib.fij_partition_offsets_layout(1, 10, 9, C.byref(chunks), C.byref(olen), C.byref(llen)))
+    assert (chunks.value, olen.value, llen.value) == (1, 512, 5120)
+    rlen = C.c_uint64()
+    # PartitionedRelation::new: len + fanout * chunks * pad
+    fij.check(fij.lib.fij_partitioned_relation_layout(1, 1000, 0, 9, 10, C.byref(chunks), C.byref(rlen), C.byref(olen)))
+    assert rlen.value == 1000 + 512 * 10 * 8 and olen.value == 5120
+    fij.check(fij.lib.fij_partitioned_relation_layout(0, 1000, 1, 9, 10, C.byref(chunks), C.byref(rlen), C.byref(olen)))
+    assert rlen.value == 1000 + 512 * 16 and olen.value == 512
+
+
+def host_relation(n, bits, chunks, kt=fij.KeyType.I64):
+    s = fij._PartitionedRelation()
+    s.relation = FAKE
+    s.offsets = FAKE
+    s.relation_len = n + (1 << bits) * chunks * fij.padding_len(kt)
+    s.len = n
+    s.chunks = chunks
+    s.radix_bits = bits
+    s.type = int(kt)
+    return s
+
+
+def test_resize_and_capacity():
+    s = host_relation(1000, 4, 2)
+    assert fij.lib.fij_partitioned_relation_capacity(C.byref(s)) == 1000
+    fij.check(fij.lib.fij_partitioned_relation_resize(C.byref(s), 600))
+    assert s.len == 600
+    assert fij.lib.fij_partitioned_relation_padded_len(C.byref(s)) == 600 + 16 * 2 * 8
+    assert fij.lib.fij_partitioned_relation_resize(C.byref(s), 1001) == Status.INVALID_ARGUMENT
+    assert s.len == 600  # an aborted resize leaves the relation intact
+
+
+def test_partition_len_host():
+    # two chunks, 4 partitions, i64 (pad 8): hand-built chunked offsets
+    s = host_relation(20, 2, 2)
+    offs = np.array([8, 19, 30, 41,  # chunk 0: sizes 3,3,3,? (end = next chunk start - pad)
+                     60, 70, 80, 90], np.uint64)
+    padded = fij.lib.fij_partitioned_relation_padded_len(C.byref(s))
+    assert padded == 20 + 4 * 2 * 8
+    out = C.c_uint64()
+    lens = []
+    for p in range(4):
+        fij.check(fij.lib.fij_partitioned_relation_partition_len(C.byref(s), offs.ctypes.data_as(C.c_void_p), p,
+                                                                 C.byref(out)))
+        lens.append(out.value)
+    # partition p: (offs[p+1]-8 - offs[p]) + (offs[4+p+1]-8 - offs[4+p]), last ends at padded_len
+    assert lens == [(19 - 8 - 8) + (70 - 8 - 60), (30 - 8 - 19) + (80 - 8 - 70), (41 - 8 - 30) + (90 - 8 - 80),
+                    (60 - 8 - 41) + (padded - 90)]
+    assert fij.lib.fij_partitioned_relation_partition_len(C.byref(s), offs.ctypes.data_as(C.c_void_p), 4,
+                                                          C.byref(out)) == Status.INVALID_ARGUMENT
+
+
+def partitioner(hist=0, algo=0, bits=(9, None, None), grid=8, block=256):
+    rb = fij.RadixBits(*bits)
+    return fij.GpuRadixPartitioner(hist, algo, rb, grid, block)
+
+
+def offsets_struct(chunks, max_chunks, bits, kt=fij.KeyType.I64, data_len=fij.LEN_UNSET, local=False):
+    s = fij._PartitionOffsets()
+    s.offsets = FAKE
+    s.local_offsets = FAKE if local else None
+    s.data_len = data_len
+    s.chunks = chunks
+    s.max_chunks = max_chunks
+    s.radix_bits = bits
+    s.type = int(kt)
+    return s
+
+
+def call_prefix_sum(p, pass_, offs, n=1000):
+    return fij.lib.fij_prefix_sum(p.h, pass_, 1, C.c_void_p(FAKE), n, C.byref(offs), None)
+
+
+def test_create_rejects_bad_block():
+    with pytest.raises(FijError) as e:
+        partitioner(block=100)
+    assert e.value.status == Status.INVALID_ARGUMENT
+
+
+def test_create_clamps_2048_block():
+    # radix-join's default block is warp_size * 32 = 2048 on 64-wide waves
+    p = partitioner(block=2048)
+    assert p.h
+
+
+def test_prefix_sum_pass_not_specified():
+    p = partitioner(bits=(9, None, None))
+    assert call_prefix_sum(p, 1, offsets_struct(8, 8, 9)) == Status.INVALID_ARGUMENT
+    assert "not specified" in fij.lib.fij_last_error().decode()
+
+
+def test_prefix_sum_mismatching_bits():
+    p = partitioner()
+    assert call_prefix_sum(p, 0, offsets_struct(8, 8, 8)) == Status.INVALID_ARGUMENT
+    assert "radix bits" in fij.lib.fij_last_error().decode()
+
+
+def test_prefix_sum_mismatching_chunks():
+    assert call_prefix_sum(partitioner(), 0, offsets_struct(7, 7, 9)) == Status.INVALID_ARGUMENT
+    contig = partitioner(hist=1)
+    assert call_prefix_sum(contig, 0, offsets_struct(8, 8, 9, local=True)) == Status.INVALID_ARGUMENT
+    assert call_prefix_sum(contig, 0, offsets_struct(1, 4, 9, local=True)) == Status.INVALID_ARGUMENT
+
+
+def test_prefix_sum_integer_overflow():
+    p = partitioner(grid=1)
+    assert call_prefix_sum(p, 0, offsets_struct(1, 1, 9), n=1 << 33) == Status.INTEGER_OVERFLOW
+
+
+def test_prefix_sum_grid_too_large_for_relation():
+    # (grid - 1) * chunk_len > len would make the last chunk negative
+    p = partitioner(grid=64)
+    assert call_prefix_sum(p, 0, offsets_struct(64, 64, 9), n=961) == Status.INVALID_ARGUMENT
+
+
+def test_too_many_radix_bits_unsupported():
+    p = partitioner(bits=(16, None, None))
+    assert call_prefix_sum(p, 0, offsets_struct(8, 8, 16)) == Status.UNSUPPORTED
+
+
+def test_partition_without_prefix_sum():
+    p = partitioner()
+    offs = offsets_struct(8, 8, 9)
+    rel = host_relation(1000, 9, 8)
+    rc = fij.lib.fij_partition(p.h, 0, 1, C.c_void_p(FAKE), C.c_void_p(FAKE), 1000, C.byref(offs), C.byref(rel), None)
+    assert rc == Status.INVALID_ARGUMENT
+    assert "no length" in fij.lib.fij_last_error().decode()
+
+
+def test_partition_mismatching_lengths_and_chunks():
+    p = partitioner()
+    offs = offsets_struct(8, 8, 9, data_len=1000)
+    rel = host_relation(999, 9, 8)
+    args = (p.h, 0, 1, C.c_void_p(FAKE), C.c_void_p(FAKE), 1000, C.byref(offs), C.byref(rel), None)
+    assert fij.lib.fij_partition(*args) == Status.INVALID_ARGUMENT
+    rel = host_relation(1000, 9, 4)
+    args = (p.h, 0, 1, C.c_void_p(FAKE), C.c_void_p(FAKE), 1000, C.byref(offs), C.byref(rel), None)
+    assert fij.lib.fij_partition(*args) == Status.INVALID_ARGUMENT
+    assert "chunks" in fij.lib.fij_last_error().decode()
+    rel = host_relation(1000, 8, 8)
+    args = (p.h, 0, 1, C.c_void_p(FAKE), C.c_void_p(FAKE), 1000, C.byref(offs), C.byref(rel), None)
+    assert fij.lib.fij_partition(*args) == Status.INVALID_ARGUMENT
+
+
+def test_transform_requires_contiguous():
+    p = partitioner(hist=0)
+    offs = offsets_struct(1, 8, 9, local=True)
+    src = host_relation(1000, 4, 8)
+    rc = fij.lib.fij_prefix_sum_and_transform(p.h, 0, 1, 0, C.byref(src), C.c_void_p(FAKE), C.c_void_p(FAKE), 10,
+                                              C.byref(offs), None)
+    assert rc == Status.INVALID_ARGUMENT
+    assert "Contiguous" in fij.lib.fij_last_error().decode()
+
+
+def test_transform_invalid_partition_id():
+    p = partitioner(hist=1, bits=(4, 9, None))
+    offs = offsets_struct(1, 8, 9, local=True)
+    src = host_relation(1000, 4, 8)
+    rc = fij.lib.fij_prefix_sum_and_transform(p.h, 1, 1, 16, C.byref(src), C.c_void_p(FAKE), C.c_void_p(FAKE), 10,
+                                              C.byref(offs), None)
+    assert rc == Status.INVALID_ARGUMENT
+    assert "partition ID" in fij.lib.fij_last_error().decode()
+
+
+def test_copy_with_payload_requires_contiguous():
+    p = partitioner(hist=0)
+    offs = offsets_struct(8, 8, 9)
+    rc = fij.lib.fij_prefix_sum_and_copy_with_payload(p.h, 0, 1, C.c_void_p(FAKE), C.c_void_p(FAKE), 100,
+                                                      C.c_void_p(FAKE), C.c_void_p(FAKE), 100, C.byref(offs), None)
+    assert rc == Status.INVALID_ARGUMENT
+
+
+def join(scheme=0, grid=4, block=128, bits=(6, None, None), pass_=0):
+    return fij.HipRadixJoin(pass_, fij.RadixBits(*bits), scheme, grid, block)
+
+
+def call_join(j, build, probe, ta_len, result_len=4 * 128):
+    return fij.lib.fij_radix_join_join(j.h, 1, C.byref(build), C.byref(probe), C.c_void_p(FAKE), result_len,
+                                       C.c_void_p(FAKE), ta_len, None, None)
+
+
+def test_join_rejects_chunked_relations():
+    j = join()
+    assert call_join(j, host_relation(100, 6, 2), host_relation(100, 6, 1), 5) == Status.INVALID_ARGUMENT
+    assert "Chunked build" in fij.lib.fij_last_error().decode()
+    assert call_join(j, host_relation(100, 6, 1), host_relation(100, 6, 2), 5) == Status.INVALID_ARGUMENT
+    assert "Chunked probe" in fij.lib.fij_last_error().decode()
+
+
+def test_join_task_assignment_length():
+    j = join()
+    assert call_join(j, host_relation(100, 6, 1), host_relation(100, 6, 1), 4) == Status.INVALID_ARGUMENT
+    assert "grid size + 1" in fij.lib.fij_last_error().decode()
+
+
+def test_join_linear_probing_unsupported():
+    j = join(scheme=1)
+    assert call_join(j, host_relation(100, 6, 1), host_relation(100, 6, 1), 5) == Status.UNSUPPORTED
+
+
+def test_join_result_set_too_small():
+    j = join()
+    assert call_join(j, host_relation(100, 6, 1), host_relation(100, 6, 1), 5, result_len=10) == \
+        Status.INVALID_ARGUMENT
+
+
+def test_join_mismatching_bits():
+    j = join()
+    assert call_join(j, host_relation(100, 5, 1), host_relation(100, 5, 1), 5) == Status.INVALID_ARGUMENT
+
+
+def test_np_build_table_too_small():
+    rc = fij.lib.fij_np_build(1, 1, 0, C.c_void_p(FAKE), 16, C.c_void_p(FAKE), C.c_void_p(FAKE), 17, 1, 256, None)
+    assert rc == Status.INVALID_ARGUMENT
+    assert "too small" in fij.lib.fij_last_error().decode()
