@@ -1,0 +1,172 @@
+"""Benchmark: plain GPU radix join at full scale on one MI355X.
+
+Workload (the reference's headline configuration for this path):
+  R = S = 2^27 tuples of (int64 key, int64 payload); R's keys are a seeded
+  permutation of 1..2^27 with payload = key; S's keys are uniform over R's
+  keys with payload = row index; two-pass radix partitioning with 9 + 9 bits,
+  then LDS hash build/probe; output {match count, SUM(R.value), SUM(S.value)}.
+
+A step is one complete join (pass-1 histograms, pass-1 scatter, pass-2,
+build/probe, aggregate read-back) over inputs already resident in HBM.
+Throughput = (|R| + |S|) / step time, as the reference defines it
+(guides/benchmarking.md:45-56).  Phase and kernel times come from HIP
+events recorded inside the timed steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--log2n 27]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "fast-interconnects_amd"))
+
+METRIC = "joined tuples/s + % HBM roofline; batched/served vs plain launch; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+SEED = 42
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=27)
+    ap.add_argument("--bits1", type=int, default=9)
+    ap.add_argument("--bits2", type=int, default=9)
+    ap.add_argument("--chunks", type=int, default=0, help="pass-1 chunks (0 = automatic)")
+    ap.add_argument("--block", type=int, default=0, help="pass-1 threads per block (0 = automatic)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-log2n", type=int, default=23, help="CPU baseline sample size (log2 of |R| = |S|)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--verify", action="store_true", help="check the result against the oracle at full size")
+    return ap.parse_args()
+
+
+def cpu_baseline(log2n, bits1, bits2, threads):
+    """The oracle's multi-threaded radix join (C restatement of the reference
+    CPU partitioner + per-partition hash join) on a bounded sample."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as orc
+
+    n = 1 << log2n
+    rk = orc.gen_perm(n, SEED, np.int64, 1)
+    rp = rk.copy()
+    sk = orc.gen_uniform(n, SEED, 1, 1, n, np.int64)
+    sp = orc.gen_iota(n, 0, np.int64)
+    orc.radix_join(rk, rp, sk, sp, bits1, bits2, threads)  # warm-up
+    times = []
+    t_end = time.time() + 15.0
+    while len(times) < 3 or (time.time() < t_end and len(times) < 10):
+        t0 = time.perf_counter()
+        res = orc.radix_join(rk, rp, sk, sp, bits1, bits2, threads)
+        times.append(time.perf_counter() - t0)
+    assert res[0] == n
+    med = float(np.median(times))
+    return {
+        "value": 2 * n / med,
+        "unit": "tuples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"R = S = 2^{log2n} int64 (same generators), {bits1}+{bits2} bits, median of {len(times)} runs",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              "(the radix join runs on one MI355X, multi-GPU sharding is design only)", file=sys.stderr)
+        sys.exit(2)
+    import fij
+
+    n = 1 << a.log2n
+    kt = fij.KeyType.I64
+    rk = fij.DeviceArray(n, np.int64)
+    rp = fij.DeviceArray(n, np.int64)
+    sk = fij.DeviceArray(n, np.int64)
+    sp = fij.DeviceArray(n, np.int64)
+    fij.gen_permutation(rk, SEED, 1)
+    fij.check(fij.lib.fij_memcpy(fij.C.c_void_p(rp.ptr), fij.C.c_void_p(rk.ptr), rk.nbytes(), None))
+    fij.gen_uniform(sk, SEED, 1, 1, n)
+    fij.gen_iota(sp, 0)
+    fij.synchronize()
+
+    plan = fij.JoinPlan(kt, n, n, a.bits1, a.bits2, pass1_chunks=a.chunks, pass1_block=a.block)
+    for _ in range(a.warmup):
+        res, _ = plan.execute(rk, rp, sk, sp)
+    phases = {"prefix_sum_ms": [], "partition_ms": [], "join_ms": [], "total_ms": []}
+    fij.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res, t = plan.execute(rk, rp, sk, sp)
+        for k in phases:
+            phases[k].append(t[k])
+    fij.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / a.steps
+    gpu_ms = float(np.median(phases["total_ms"]))
+    # expected aggregate: every S key matches exactly one R key (payload = key)
+    assert res[0] == n, f"match count {res[0]} != {n}"
+
+    value = 2 * n / (ms_per_step / 1000.0)
+    t_bytes = 16
+    b_alg = 2 * n * (8 + (2 * 2 + 1) * t_bytes)  # (|R|+|S|) * (k + (2P+1) t), P = 2
+    # dominant kernel: the pass-1 scatter (two launches per step, one per relation)
+    scatter_ms = float(np.median(phases["partition_ms"])) / 2
+    scatter_bytes = n * (8 + 8 + t_bytes)  # read key + payload, write one tuple
+    achieved = scatter_bytes / (scatter_ms / 1000.0) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded on-device generators)",
+        "config": {
+            "workload": f"plain radix join, R = S = 2^{a.log2n} (int64 key, int64 payload), "
+                        f"{a.bits1}+{a.bits2} radix bits, LDS hash build/probe, count + sums",
+            "build_tuples": n,
+            "probe_tuples": n,
+            "radix_bits": [a.bits1, a.bits2],
+            "parallelism": "single GPU",
+        },
+        "gpu_ms_per_join": gpu_ms,
+        "phases_ms": {k: float(np.median(v)) for k, v in phases.items()},
+        "pipeline_roofline_frac": b_alg / (gpu_ms / 1000.0) / (HBM_PEAK_GBS * 1e9),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "scatter_swwc_kernel (pass-1 scatter, one relation per launch)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "result": {"count": res[0], "sum_build": res[1], "sum_probe": res[2]},
+    }
+    if a.verify:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib as orc
+
+        want = orc.radix_join(rk.to_host(), rp.to_host(), sk.to_host(), sp.to_host(), a.bits1, a.bits2,
+                              threads=os.cpu_count() or 8)
+        out["verified"] = tuple(res) == tuple(want)
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_log2n, a.bits1, a.bits2, threads)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -663,7 +663,14 @@ fij_status fij_radix_join_join(fij_radix_join* j, fij_type type, const fij_parti
 fij_status fij_np_table_init(fij_type type, void* table, uint64_t entries, void* stream) {
   if (!valid_type(type) || (entries && !table)) return fail(FIJ_INVALID_ARGUMENT, "invalid argument");
   if (entries == 0) return FIJ_OK;
-  FIJ_HIP(launch_np_table_init(key_type(type), table, entries, static_cast<hipStream_t>(stream)));
+  FIJ_HIP(launch_np_table_init(key_type(type), table, entries, true, static_cast<hipStream_t>(stream)));
+  return FIJ_OK;
+}
+
+fij_status fij_relation_fill_null(fij_type type, void* relation, uint64_t tuples, void* stream) {
+  if (!valid_type(type) || (tuples && !relation)) return fail(FIJ_INVALID_ARGUMENT, "invalid argument");
+  if (tuples == 0) return FIJ_OK;
+  FIJ_HIP(launch_np_table_init(key_type(type), relation, tuples, false, static_cast<hipStream_t>(stream)));
   return FIJ_OK;
 }
 

@@ -107,7 +107,8 @@ hipError_t launch_assign_tasks(const uint64_t* probe_offsets, uint64_t probe_len
 hipError_t launch_join(const JoinLaunch& a, hipStream_t stream);
 
 // ---- no-partitioning join (np_join.hip) -------------------------------------
-hipError_t launch_np_table_init(KeyType t, void* table, uint64_t entries, hipStream_t stream);
+hipError_t launch_np_table_init(KeyType t, void* table, uint64_t entries, bool null_values,
+                                hipStream_t stream);
 hipError_t launch_np_build(KeyType t, JoinScheme s, bool selective, void* table, uint64_t entries,
                            const void* keys, const void* pays, uint64_t n, uint32_t grid,
                            uint32_t block, hipStream_t stream);

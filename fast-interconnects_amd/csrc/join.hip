@@ -122,6 +122,7 @@ __global__ __launch_bounds__(1024) void join_perfect_kernel(JoinParams p) {
     __syncthreads();
     for (uint64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
       const Tuple<K> t = load_tuple<K>(build + i);
+      if (t.key == null_key<K>()) continue;  // null keys never join
       const uint32_t idx = perfect_index<K>(t.key, p.ignore_bits);
       if (idx < E) ht[idx] = t;
       else overflow = true;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(1024) void join_perfect_kernel(JoinParams p) {
     for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
       const Tuple<K> t = load_tuple<K>(probe + i);
       const uint32_t idx = perfect_index<K>(t.key, p.ignore_bits);
-      if (idx < E && ht[idx].key == t.key) {
+      if (t.key != null_key<K>() && idx < E && ht[idx].key == t.key) {
         cnt += 1;
         sb += static_cast<uint64_t>(static_cast<int64_t>(ht[idx].value));
         sp += static_cast<uint64_t>(static_cast<int64_t>(t.value));
@@ -173,12 +174,14 @@ __global__ __launch_bounds__(1024) void join_chaining_kernel(JoinParams p) {
       const Tuple<K> t = load_tuple<K>(build + b0 + i);
       keys[i] = t.key;
       vals[i] = t.value;
+      if (t.key == null_key<K>()) continue;  // stored but never linked: null keys never join
       const uint32_t bucket = mult_shift_hash32(perfect_index<K>(t.key, p.ignore_bits), log2_buckets);
       links[i] = static_cast<uint16_t>(atomicExch(&heads[bucket], i));
     }
     __syncthreads();
     for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
       const Tuple<K> t = load_tuple<K>(probe + i);
+      if (t.key == null_key<K>()) continue;
       const uint32_t bucket = mult_shift_hash32(perfect_index<K>(t.key, p.ignore_bits), log2_buckets);
       for (uint32_t e = heads[bucket]; e != kChainTail; e = links[e]) {
         if (keys[e] == t.key) {

@@ -14,10 +14,10 @@ namespace fij {
 namespace {
 
 template <typename K>
-__global__ void np_init_kernel(Tuple<K>* table, uint64_t entries) {
+__global__ void np_init_kernel(Tuple<K>* table, uint64_t entries, K value) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < entries; i += stride)
-    store_tuple<K>(table + i, null_key<K>(), K(0));
+    store_tuple<K>(table + i, null_key<K>(), value);
 }
 
 __device__ __forceinline__ uint32_t log2_floor(uint64_t x) { return 63u - __builtin_clzll(x); }
@@ -51,8 +51,10 @@ template <typename K>
 __global__ __launch_bounds__(1024) void np_build_lp_kernel(Tuple<K>* table, uint64_t entries, const K* keys, const K* pays, uint64_t n) {
   const uint32_t lg = log2_floor(entries);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    lp_insert<K>(table, lg, keys[i], pays[i]);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const K key = keys[i];
+    if (key != null_key<K>()) lp_insert<K>(table, lg, key, pays[i]);  // null keys never join
+  }
 }
 
 template <typename K>
@@ -118,6 +120,7 @@ __global__ __launch_bounds__(1024) void np_probe_kernel(const Tuple<K>* table, u
     for (uint64_t i = gid; i < n; i += stride) {
       const K key = keys[i];
       const K pay = pays[i];
+      if (key == null_key<K>()) continue;  // would match every empty slot
       uint64_t idx = mult_shift_hash<K>(key, lg);
       for (uint64_t a = 0; a <= mask; ++a, idx = (idx + 1) & mask) {
         const Tuple<K> e = load_tuple<K>(table + idx);
@@ -165,14 +168,18 @@ hipError_t probe_t(JoinScheme s, const void* table, uint64_t entries, const void
 
 }  // namespace
 
-hipError_t launch_np_table_init(KeyType t, void* table, uint64_t entries, hipStream_t stream) {
+// null_values: every word of the entry becomes the null key, as the
+// reference's cuMemsetD32 initialisation does (no_partitioning_join.rs:
+// 611-624) -- the i32 build's 64-bit CAS expects that; otherwise the payload
+// is zeroed (the reference tests' Tuple { null_key, default } padding).
+hipError_t launch_np_table_init(KeyType t, void* table, uint64_t entries, bool null_values, hipStream_t stream) {
   const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((entries + 255) / 256, 8192));
   if (t == KeyType::I32)
     hipLaunchKernelGGL(np_init_kernel<int32_t>, dim3(grid ? grid : 1), dim3(256), 0, stream,
-                       static_cast<Tuple<int32_t>*>(table), entries);
+                       static_cast<Tuple<int32_t>*>(table), entries, null_values ? null_key<int32_t>() : 0);
   else
     hipLaunchKernelGGL(np_init_kernel<int64_t>, dim3(grid ? grid : 1), dim3(256), 0, stream,
-                       static_cast<Tuple<int64_t>*>(table), entries);
+                       static_cast<Tuple<int64_t>*>(table), entries, null_values ? null_key<int64_t>() : 0);
   return hipGetLastError();
 }
 

@@ -1,0 +1,51 @@
+// Device-visible state of the batched radix-join pipeline (pipeline.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "internal.hpp"
+
+namespace fij {
+
+// One relation's buffers.  rel1 is the pass-1 partitioned relation in the
+// chunked layout (G1 chunks x F1 partitions, 128-byte padding in front of
+// every partition) with offsets off1[G1][F1]; rel2 holds the pass-2 output:
+// the concatenation of all pass-1 partitions, each cut into tiles of tile2
+// tuples that are sorted by the pass-2 digit.
+struct PipelineSide {
+  const void* keys;           // input columns
+  const void* pays;
+  uint64_t n;
+  uint64_t* off1;             // [G1 * F1]
+  void* rel1;                 // Tuple[n + F1 * G1 * pad]
+  uint64_t padded_len1;
+  uint32_t* piece_start;      // [F1][G1 + 1]
+  uint32_t* plen;             // [F1]
+  uint64_t* pstart;           // [F1 + 1] start of partition p in rel2
+  uint32_t* tile_start;       // [F1 + 1] first pass-2 tile of partition p
+  void* rel2;                 // Tuple[n]
+  uint16_t* meta;             // [tiles][F2 + 1] run starts per tile
+};
+
+struct PipelineDevice {
+  PipelineSide build;
+  PipelineSide probe;
+  uint32_t F1, F2, G1;
+  uint32_t bits1, bits2;
+  uint32_t pad;
+  uint32_t tile2;
+  uint32_t* work_counter;
+  JoinAggregate* aggregates;  // [agg_len]
+  uint32_t agg_len;
+};
+
+uint32_t pipeline_tile2(KeyType t);
+size_t pipeline_sort2_lds(KeyType t, uint32_t F2, uint32_t G1);
+hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
+hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t rel, uint32_t max_tiles,
+                                 hipStream_t stream);
+hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, uint32_t grid,
+                                hipStream_t stream);
+
+}  // namespace fij

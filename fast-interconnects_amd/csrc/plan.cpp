@@ -1,0 +1,286 @@
+// Host side of the batched radix-join execution method (fij_join_plan_*).
+//
+// A plan owns every intermediate buffer of one join shape (allocated once,
+// as the reference allocates its partitions and stream states before the
+// timed phases: radix-join/src/execution_methods/gpu_radix_join.rs:153-205,
+// :307-396), so that executing it launches kernels only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/fij.h"
+#include "internal.hpp"
+#include "pipeline.hpp"
+
+namespace fij {
+fij_status set_error(fij_status s, const char* msg);
+}
+
+using namespace fij;
+
+struct fij_join_plan {
+  fij_join_plan_config cfg;
+  KeyType kt;
+  PipelineDevice d;
+  uint32_t max_tiles[2];
+  uint32_t join_grid;
+  uint32_t block1;
+  uint32_t lds_total;
+  bool large_tables;
+  std::vector<void*> allocs;
+  hipEvent_t ev[5];
+  JoinAggregate* host_agg;
+};
+
+namespace {
+
+fij_status plan_fail(fij_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  return set_error(s, buf);
+}
+
+fij_status hip_fail(hipError_t e, const char* what) {
+  return plan_fail(e == hipErrorOutOfMemory ? FIJ_OUT_OF_MEMORY : FIJ_DEVICE_ERROR, "%s: %s", what,
+                   hipGetErrorString(e));
+}
+
+#define PLAN_HIP(expr)                                \
+  do {                                                \
+    hipError_t e_ = (expr);                           \
+    if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+  } while (0)
+
+template <typename T>
+fij_status alloc(fij_join_plan* p, T** out, uint64_t count) {
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, std::max<uint64_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  p->allocs.push_back(ptr);
+  *out = static_cast<T*>(ptr);
+  return FIJ_OK;
+}
+
+void release(fij_join_plan* p) {
+  for (void* a : p->allocs) (void)hipFree(a);
+  p->allocs.clear();
+  for (hipEvent_t& e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->host_agg) (void)hipHostFree(p->host_agg);
+}
+
+fij_status alloc_side(fij_join_plan* p, PipelineSide& s, uint64_t n, uint32_t tiles) {
+  const uint32_t F1 = p->d.F1, G1 = p->d.G1, F2 = p->d.F2;
+  const uint32_t tb = tuple_bytes(p->kt);
+  s.n = n;
+  s.padded_len1 = n + static_cast<uint64_t>(F1) * G1 * p->d.pad;
+  fij_status st;
+  if ((st = alloc(p, &s.off1, static_cast<uint64_t>(F1) * G1))) return st;
+  uint8_t* rel1 = nullptr;
+  if ((st = alloc(p, &rel1, s.padded_len1 * tb))) return st;
+  s.rel1 = rel1;
+  if ((st = alloc(p, &s.piece_start, static_cast<uint64_t>(F1) * (G1 + 1)))) return st;
+  if ((st = alloc(p, &s.plen, F1))) return st;
+  if ((st = alloc(p, &s.pstart, F1 + 1))) return st;
+  if ((st = alloc(p, &s.tile_start, F1 + 1))) return st;
+  if (p->d.bits2) {
+    uint8_t* rel2 = nullptr;
+    if ((st = alloc(p, &rel2, n * tb))) return st;
+    s.rel2 = rel2;
+    if ((st = alloc(p, &s.meta, static_cast<uint64_t>(tiles) * (F2 + 1)))) return st;
+  } else {
+    s.rel2 = nullptr;
+    s.meta = nullptr;
+  }
+  return FIJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan** out) {
+  if (!cfg || !out) return plan_fail(FIJ_INVALID_ARGUMENT, "NULL argument");
+  if (cfg->type != FIJ_TYPE_I32 && cfg->type != FIJ_TYPE_I64) return plan_fail(FIJ_INVALID_ARGUMENT, "unknown type");
+  if (cfg->radix_bits_1 < 1 || cfg->radix_bits_1 > 12)
+    return plan_fail(FIJ_UNSUPPORTED, "pass-1 radix bits must be in [1, 12]");
+  if (cfg->radix_bits_2 > 12) return plan_fail(FIJ_UNSUPPORTED, "pass-2 radix bits must be in [0, 12]");
+  const KeyType kt = cfg->type == FIJ_TYPE_I32 ? KeyType::I32 : KeyType::I64;
+  if (cfg->radix_bits_1 + cfg->radix_bits_2 > 8 * key_bytes(kt))
+    return plan_fail(FIJ_INVALID_ARGUMENT, "more radix bits than key bits");
+  int dev = 0;
+  PLAN_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  PLAN_HIP(hipGetDeviceProperties(&prop, dev));
+  int optin = 0;
+  uint32_t lds_total = static_cast<uint32_t>(prop.sharedMemPerBlock);
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > 0)
+    lds_total = std::max<uint32_t>(lds_total, static_cast<uint32_t>(optin));
+  const uint32_t cus = static_cast<uint32_t>(prop.multiProcessorCount);
+
+  fij_join_plan* p = new fij_join_plan();
+  std::memset(p->ev, 0, sizeof(p->ev));
+  p->host_agg = nullptr;
+  p->cfg = *cfg;
+  p->kt = kt;
+  p->lds_total = lds_total;
+  std::memset(&p->d, 0, sizeof(p->d));
+  p->d.bits1 = cfg->radix_bits_1;
+  p->d.bits2 = cfg->radix_bits_2;
+  p->d.F1 = 1u << cfg->radix_bits_1;
+  p->d.F2 = 1u << cfg->radix_bits_2;
+  p->d.G1 = cfg->pass1_chunks ? cfg->pass1_chunks : cus;
+  p->d.G1 += p->d.G1 & 1;  // even, keeps the sort's LDS piece table 8-byte aligned
+  p->d.pad = pad_tuples(kt);
+  p->d.tile2 = pipeline_tile2(kt);
+  p->block1 = cfg->pass1_block ? std::min<uint32_t>(cfg->pass1_block, 1024) : 1024;
+  p->large_tables = cfg->radix_bits_2 == 0;
+  p->join_grid = cfg->join_blocks ? cfg->join_blocks : cus * 2;
+  p->d.agg_len = p->join_grid;
+  fij_status st = FIJ_OK;
+  const uint64_t lens[2] = {cfg->build_len, cfg->probe_len};
+  for (int rel = 0; rel < 2 && st == FIJ_OK; ++rel) {
+    uint64_t chunk = 0;
+    st = fij_input_chunk_size(static_cast<fij_type>(cfg->type), lens[rel], p->d.G1, &chunk);
+    if (st == FIJ_OK && chunk * (p->d.G1 - 1) > lens[rel])
+      st = plan_fail(FIJ_INVALID_ARGUMENT, "relation of %llu tuples is too short for %u chunks",
+                     static_cast<unsigned long long>(lens[rel]), p->d.G1);
+    const uint64_t last = lens[rel] - chunk * (p->d.G1 - 1);
+    if (st == FIJ_OK && std::max(chunk, last) + static_cast<uint64_t>(p->d.F1) * p->d.pad * 2 >= 0xFFFFFFFFull)
+      st = plan_fail(FIJ_INTEGER_OVERFLOW, "pass-1 chunks exceed 32-bit offsets; use more chunks");
+    p->max_tiles[rel] = p->d.bits2 ? static_cast<uint32_t>((lens[rel] + p->d.tile2 - 1) / p->d.tile2 + p->d.F1) : 0;
+  }
+  if (st == FIJ_OK && p->d.bits2 && pipeline_sort2_lds(kt, p->d.F2, p->d.G1) > lds_total)
+    st = plan_fail(FIJ_UNSUPPORTED, "pass-2 tile does not fit LDS for this fanout / chunk count");
+  if (st == FIJ_OK) st = alloc_side(p, p->d.build, cfg->build_len, p->max_tiles[0]);
+  if (st == FIJ_OK) st = alloc_side(p, p->d.probe, cfg->probe_len, p->max_tiles[1]);
+  if (st == FIJ_OK) st = alloc(p, &p->d.work_counter, 1);
+  if (st == FIJ_OK) st = alloc(p, &p->d.aggregates, p->join_grid);
+  for (int i = 0; i < 5 && st == FIJ_OK; ++i)
+    if (hipEventCreate(&p->ev[i]) != hipSuccess) st = plan_fail(FIJ_DEVICE_ERROR, "hipEventCreate failed");
+  if (st == FIJ_OK && hipHostMalloc(reinterpret_cast<void**>(&p->host_agg), sizeof(JoinAggregate) * p->join_grid,
+                                    hipHostMallocDefault) != hipSuccess)
+    st = plan_fail(FIJ_OUT_OF_MEMORY, "cannot allocate pinned aggregates");
+  if (st != FIJ_OK) {
+    release(p);
+    delete p;
+    return st;
+  }
+  *out = p;
+  return FIJ_OK;
+}
+
+fij_status fij_join_plan_destroy(fij_join_plan* plan) {
+  if (!plan) return FIJ_OK;
+  release(plan);
+  delete plan;
+  return FIJ_OK;
+}
+
+fij_status fij_join_plan_enqueue(fij_join_plan* p, const void* build_keys, const void* build_pays,
+                                 const void* probe_keys, const void* probe_pays, void* stream) {
+  if (!p) return plan_fail(FIJ_INVALID_ARGUMENT, "plan is NULL");
+  if ((p->d.build.n && (!build_keys || !build_pays)) || (p->d.probe.n && (!probe_keys || !probe_pays)))
+    return plan_fail(FIJ_INVALID_ARGUMENT, "NULL input column");
+  const size_t kb = key_bytes(p->kt);
+  for (const void* c : {build_keys, build_pays, probe_keys, probe_pays})
+    if (c && reinterpret_cast<uintptr_t>(c) % kb)
+      return plan_fail(FIJ_INVALID_ARGUMENT, "input columns must be aligned to their element size");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  PipelineDevice& d = p->d;
+  d.build.keys = build_keys;
+  d.build.pays = build_pays;
+  d.probe.keys = probe_keys;
+  d.probe.pays = probe_pays;
+  PLAN_HIP(hipEventRecord(p->ev[0], st));
+  for (PipelineSide* s : {&d.build, &d.probe})
+    PLAN_HIP(launch_hist_chunked(p->kt, s->keys, s->n, d.G1, d.bits1, 0, s->off1, p->block1, st));
+  PLAN_HIP(hipEventRecord(p->ev[1], st));
+  for (PipelineSide* s : {&d.build, &d.probe}) {
+    PartitionLaunch a{};
+    a.type = p->kt;
+    a.keys = s->keys;
+    a.pays = s->pays;
+    a.n = s->n;
+    a.chunks = d.G1;
+    a.radix_bits = d.bits1;
+    a.ignore_bits = 0;
+    a.chunk_offsets = s->off1;
+    a.relation = s->rel1;
+    a.block = p->block1;
+    a.lds_bytes = p->lds_total;
+    a.buffered = true;
+    PLAN_HIP(launch_partition(a, st));
+  }
+  PLAN_HIP(hipEventRecord(p->ev[2], st));
+  PLAN_HIP(launch_pipeline_plan(d, st));
+  if (d.bits2) {
+    PLAN_HIP(launch_pipeline_sort2(d, p->kt, 0, p->max_tiles[0], st));
+    PLAN_HIP(launch_pipeline_sort2(d, p->kt, 1, p->max_tiles[1], st));
+  }
+  PLAN_HIP(hipEventRecord(p->ev[3], st));
+  PLAN_HIP(launch_pipeline_join(d, p->kt, p->large_tables, p->join_grid, st));
+  PLAN_HIP(hipEventRecord(p->ev[4], st));
+  PLAN_HIP(hipMemcpyAsync(p->host_agg, d.aggregates, sizeof(JoinAggregate) * p->join_grid, hipMemcpyDeviceToHost,
+                          st));
+  return FIJ_OK;
+}
+
+fij_status fij_join_plan_result(fij_join_plan* p, void* stream, fij_join_aggregate* result, fij_join_timing* timing) {
+  if (!p) return plan_fail(FIJ_INVALID_ARGUMENT, "plan is NULL");
+  PLAN_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  if (result) {
+    uint64_t c = 0, b = 0, s = 0;
+    for (uint32_t i = 0; i < p->join_grid; ++i) {
+      c += p->host_agg[i].count;
+      b += p->host_agg[i].sum_build;
+      s += p->host_agg[i].sum_probe;
+    }
+    result->count = c;
+    result->sum_build = b;
+    result->sum_probe = s;
+  }
+  if (timing) {
+    float a = 0, b = 0, c = 0, t = 0;
+    PLAN_HIP(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
+    PLAN_HIP(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
+    PLAN_HIP(hipEventElapsedTime(&c, p->ev[2], p->ev[4]));
+    PLAN_HIP(hipEventElapsedTime(&t, p->ev[0], p->ev[4]));
+    timing->prefix_sum_ms = a;
+    timing->partition_ms = b;
+    timing->join_ms = c;
+    timing->total_ms = t;
+  }
+  return FIJ_OK;
+}
+
+fij_status fij_join_plan_execute(fij_join_plan* p, const void* build_keys, const void* build_pays,
+                                 const void* probe_keys, const void* probe_pays, fij_join_aggregate* result,
+                                 fij_join_timing* timing, void* stream) {
+  fij_status s = fij_join_plan_enqueue(p, build_keys, build_pays, probe_keys, probe_pays, stream);
+  if (s != FIJ_OK) return s;
+  return fij_join_plan_result(p, stream, result, timing);
+}
+
+fij_status fij_join_plan_device_bytes(const fij_join_plan* p, uint64_t* bytes) {
+  if (!p || !bytes) return plan_fail(FIJ_INVALID_ARGUMENT, "NULL argument");
+  const uint64_t tb = tuple_bytes(p->kt);
+  uint64_t total = 0;
+  for (const PipelineSide* s : {&p->d.build, &p->d.probe}) {
+    total += s->padded_len1 * tb + static_cast<uint64_t>(p->d.F1) * p->d.G1 * 8;
+    total += static_cast<uint64_t>(p->d.F1) * (p->d.G1 + 1) * 4 + p->d.F1 * 4 + (p->d.F1 + 1) * 12;
+    if (p->d.bits2) total += s->n * tb;
+  }
+  total += static_cast<uint64_t>(p->max_tiles[0] + p->max_tiles[1]) * (p->d.F2 + 1) * 2;
+  *bytes = total;
+  return FIJ_OK;
+}
+
+}  // extern "C"

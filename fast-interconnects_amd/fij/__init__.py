@@ -159,6 +159,7 @@ _proto("fij_radix_join_destroy", C.c_int, _vp)
 _proto("fij_radix_join_join", C.c_int, _vp, C.c_int, C.POINTER(_PartitionedRelation), C.POINTER(_PartitionedRelation), _vp, _u64, _vp, _u64, _vp, _vp)
 _proto("fij_join_assign_tasks", C.c_int, C.POINTER(_PartitionedRelation), _u32, _u32, _vp, _vp)
 _proto("fij_np_table_init", C.c_int, C.c_int, _vp, _u64, _vp)
+_proto("fij_relation_fill_null", C.c_int, C.c_int, _vp, _u64, _vp)
 _proto("fij_np_build", C.c_int, C.c_int, C.c_int, C.c_int, _vp, _u64, _vp, _vp, _u64, _u32, _u32, _vp)
 _proto("fij_np_probe_sum", C.c_int, C.c_int, C.c_int, _vp, _u64, _vp, _vp, _u64, _vp, _u64, _vp, _u32, _u32, _vp)
 _proto("fij_gen_permutation", C.c_int, C.c_int, _vp, _u64, _u64, _i64, _vp)
@@ -513,7 +514,7 @@ class PartitionedRelation:
 
     def fill_null(self, stream=None):
         """Sets every tuple to {null key, 0} (as the reference's tests do)."""
-        check(lib.fij_np_table_init(int(self.ktype), _ptr(self.relation), self.s.relation_len, _stream(stream)))
+        check(lib.fij_relation_fill_null(int(self.ktype), _ptr(self.relation), self.s.relation_len, _stream(stream)))
 
     def partition_len(self, pid, host_offsets=None):
         h = self.host_offsets() if host_offsets is None else host_offsets
@@ -692,3 +693,78 @@ def gen_uniform(out, seed, stream_id, lo, count, stream=None):
 
 def gen_iota(out, base=0, stream=None):
     check(lib.fij_gen_iota(int(_ktype(out.dtype)), _ptr(out), out.numel(), base, _stream(stream)))
+
+
+# --------------------------------------------------------------------------
+# radix-join execution method (include/fij.h: fij_join_plan_*)
+
+
+class _PlanConfig(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("radix_bits_1", C.c_uint32),
+        ("radix_bits_2", C.c_uint32),
+        ("pass1_chunks", C.c_uint32),
+        ("pass1_block", C.c_uint32),
+        ("join_blocks", C.c_uint32),
+        ("build_len", C.c_uint64),
+        ("probe_len", C.c_uint64),
+    ]
+
+
+class _Timing(C.Structure):
+    _fields_ = [("prefix_sum_ms", C.c_float), ("partition_ms", C.c_float), ("join_ms", C.c_float),
+                ("total_ms", C.c_float)]
+
+
+_proto("fij_join_plan_create", C.c_int, C.POINTER(_PlanConfig), C.POINTER(_vp))
+_proto("fij_join_plan_destroy", C.c_int, _vp)
+_proto("fij_join_plan_device_bytes", C.c_int, _vp, C.POINTER(_u64))
+_proto("fij_join_plan_enqueue", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
+_proto("fij_join_plan_result", C.c_int, _vp, _vp, C.POINTER(JoinAggregate), C.POINTER(_Timing))
+_proto("fij_join_plan_execute", C.c_int, _vp, _vp, _vp, _vp, _vp, C.POINTER(JoinAggregate), C.POINTER(_Timing), _vp)
+
+
+class JoinPlan:
+    """Radix join of R (build) and S (probe): pass 1 with bits1, pass 2 with
+    bits2 (0 = single pass), LDS hash build/probe; returns
+    (match count, SUM(R.value), SUM(S.value)) as wrapping u64."""
+
+    def __init__(self, ktype, build_len, probe_len, bits1, bits2, pass1_chunks=0, pass1_block=0, join_blocks=0):
+        cfg = _PlanConfig()
+        cfg.type = int(_ktype(ktype))
+        cfg.radix_bits_1 = bits1
+        cfg.radix_bits_2 = bits2
+        cfg.pass1_chunks = pass1_chunks
+        cfg.pass1_block = pass1_block
+        cfg.join_blocks = join_blocks
+        cfg.build_len = build_len
+        cfg.probe_len = probe_len
+        h = _vp()
+        check(lib.fij_join_plan_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.fij_join_plan_destroy(self.h)
+            self.h = None
+
+    def device_bytes(self):
+        out = _u64()
+        check(lib.fij_join_plan_device_bytes(self.h, C.byref(out)))
+        return out.value
+
+    def enqueue(self, bk, bp, sk, sp, stream=None):
+        check(lib.fij_join_plan_enqueue(self.h, _ptr(bk), _ptr(bp), _ptr(sk), _ptr(sp), _stream(stream)))
+
+    def result(self, stream=None):
+        r = JoinAggregate()
+        t = _Timing()
+        check(lib.fij_join_plan_result(self.h, _stream(stream), C.byref(r), C.byref(t)))
+        return (r.count, r.sum_build, r.sum_probe), {
+            "prefix_sum_ms": t.prefix_sum_ms, "partition_ms": t.partition_ms, "join_ms": t.join_ms,
+            "total_ms": t.total_ms}
+
+    def execute(self, bk, bp, sk, sp, stream=None):
+        self.enqueue(bk, bp, sk, sp, stream)
+        return self.result(stream)

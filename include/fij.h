@@ -259,8 +259,13 @@ fij_status fij_join_assign_tasks(const fij_partitioned_relation *probe_rel, uint
 /* ---------------------------------------------- no-partitioning join ---- */
 
 /* HashTable::new_on_gpu (sql-ops/src/join/no_partitioning_join.rs:588-634):
- * sets every entry to {null key, 0}. */
+ * sets every 32-bit word of the table to the null key, as the reference's
+ * cuMemsetD32 does. */
 fij_status fij_np_table_init(fij_type type, void *table, uint64_t entries, void *stream);
+/* Sets every tuple of a relation buffer to {null key, 0}, the padding value
+ * the reference's partition tests allocate with
+ * (sql-ops/tests/test_gpu_radix_partition.rs:88-97). */
+fij_status fij_relation_fill_null(fij_type type, void *relation, uint64_t tuples, void *stream);
 /* CudaHashJoin::build (no_partitioning_join.rs:255-268, impl :313-375; kernels
  * no_partitioning_join.cu:115-132, :251-319).  is_selective skips null
  * build keys (perfect hashing only). */
@@ -275,6 +280,54 @@ fij_status fij_np_probe_sum(fij_type type, fij_hashing_scheme scheme, const void
                             const void *keys, const void *payloads, uint64_t len, uint64_t *result_set,
                             uint64_t result_set_len, fij_join_aggregate *aggregates, uint32_t grid_size,
                             uint32_t block_size, void *stream);
+
+/* ----------------------------------------- radix-join execution method ---- */
+/* The radix-join binary's GpuRadixJoinTwoPass execution method
+ * (radix-join/src/execution_methods/gpu_radix_join.rs:90-565), which the
+ * reference builds from the operators above with a host loop of six launches
+ * per pass-1 partition.  A plan allocates every intermediate buffer of one
+ * join shape once; executing it runs pass 1 (chunked histogram + buffered
+ * scatter), pass 2 (per-partition LDS counting sort of tiles) and the LDS
+ * hash join for all partitions with a fixed, small number of launches, and
+ * returns {count, SUM(build.value), SUM(probe.value)}. */
+
+typedef struct fij_join_plan fij_join_plan;
+
+typedef struct fij_join_plan_config {
+  uint32_t type;         /* fij_type */
+  uint32_t radix_bits_1; /* pass-1 bits (RadixBits first pass) */
+  uint32_t radix_bits_2; /* pass-2 bits; 0 = single pass */
+  uint32_t pass1_chunks; /* pass-1 chunks (= blocks); 0 = one per compute unit */
+  uint32_t pass1_block;  /* threads per pass-1 block; 0 = 1024 */
+  uint32_t join_blocks;  /* resident blocks of the join; 0 = two per compute unit */
+  uint64_t build_len;    /* |R| */
+  uint64_t probe_len;    /* |S| */
+} fij_join_plan_config;
+
+/* Phase times as in the reference's RadixJoinPoint
+ * (radix-join/src/measurement/data_point.rs:29-74): prefix sum (pass-1
+ * histograms), partition (pass-1 scatter), join (pass 2 + build/probe). */
+typedef struct fij_join_timing {
+  float prefix_sum_ms;
+  float partition_ms;
+  float join_ms;
+  float total_ms;
+} fij_join_timing;
+
+fij_status fij_join_plan_create(const fij_join_plan_config *config, fij_join_plan **out);
+fij_status fij_join_plan_destroy(fij_join_plan *plan);
+/* Device bytes the plan holds (partitions, offsets, tile metadata). */
+fij_status fij_join_plan_device_bytes(const fij_join_plan *plan, uint64_t *bytes);
+/* Enqueues one join on `stream`; the result is read with fij_join_plan_result. */
+fij_status fij_join_plan_enqueue(fij_join_plan *plan, const void *build_keys, const void *build_payloads,
+                                 const void *probe_keys, const void *probe_payloads, void *stream);
+/* Waits for the enqueued join; fills result and (optionally) phase times. */
+fij_status fij_join_plan_result(fij_join_plan *plan, void *stream, fij_join_aggregate *result,
+                                fij_join_timing *timing);
+/* enqueue + result. */
+fij_status fij_join_plan_execute(fij_join_plan *plan, const void *build_keys, const void *build_payloads,
+                                 const void *probe_keys, const void *probe_payloads, fij_join_aggregate *result,
+                                 fij_join_timing *timing, void *stream);
 
 /* ---------------------------------------------------- data generation ---- */
 /* Seeded replacements for the reference's unseeded generators

@@ -350,6 +350,7 @@ static uint64_t next_pow2(uint64_t x) {
     const uint32_t lg = (uint32_t)__builtin_ctzll(cap);                                  \
     for (uint64_t i = 0; i < cap; ++i) slots[i] = 0xFFFFFFFFu;                           \
     for (uint64_t i = 0; i < nb; ++i) {                                                  \
+      if (build[i].key == (K)-1) continue; /* null keys never join */                    \
       uint64_t h = (uint64_t)orc_mult_shift_hash_i64((int64_t)build[i].key, lg);         \
       while (slots[h] != 0xFFFFFFFFu) h = (h + 1) & (cap - 1);                           \
       slots[h] = (uint32_t)i;                                                            \
@@ -357,6 +358,7 @@ static uint64_t next_pow2(uint64_t x) {
     uint64_t cnt = 0, sr = 0, ss = 0;                                                    \
     for (uint64_t j = 0; j < np; ++j) {                                                  \
       K key = probe[j].key;                                                              \
+      if (key == (K)-1) continue;                                                        \
       uint64_t h = (uint64_t)orc_mult_shift_hash_i64((int64_t)key, lg);                  \
       while (slots[h] != 0xFFFFFFFFu) {                                                  \
         const T *b = &build[slots[h]];                                                   \
@@ -419,13 +421,14 @@ static uint64_t next_pow2(uint64_t x) {
       uint64_t p0 = poff[p], p1 = part_end_##SUF(poff, F, p, pplen, pad);                \
       if (b1 - b0 > ht_entries) rc = -1;                                                 \
       for (uint64_t i = b0; i < b1; ++i) {                                               \
+        if (build[i].key == (K)-1) continue;                                             \
         uint64_t idx = (uint32_t)((uint64_t)(int64_t)build[i].key >> ignore_bits);       \
         if (idx < ht_entries) ht[idx] = build[i];                                        \
         else rc = -1;                                                                    \
       }                                                                                  \
       for (uint64_t j = p0; j < p1; ++j) {                                               \
         uint64_t idx = (uint32_t)((uint64_t)(int64_t)probe[j].key >> ignore_bits);       \
-        if (idx < ht_entries && ht[idx].key == probe[j].key) {                           \
+        if (probe[j].key != (K)-1 && idx < ht_entries && ht[idx].key == probe[j].key) {  \
           res->count += 1;                                                               \
           res->sum_build += (uint64_t)(int64_t)ht[idx].value;                            \
           res->sum_probe += (uint64_t)(int64_t)probe[j].value;                           \
@@ -443,6 +446,7 @@ static uint64_t next_pow2(uint64_t x) {
     const uint32_t lg = 63u - (uint32_t)__builtin_clzll(entries);                        \
     const uint64_t mask = (1ULL << lg) - 1ULL;                                           \
     for (uint64_t i = 0; i < n; ++i) {                                                   \
+      if (keys[i] == (K)-1) continue; /* null keys never join */                         \
       uint64_t idx = (uint64_t)orc_mult_shift_hash_##SUF(keys[i], lg);                   \
       for (uint64_t a = 0; a <= mask; ++a, idx = (idx + 1) & mask) {                     \
         if (table[idx].key == (K)-1) {                                                   \
@@ -459,6 +463,7 @@ static uint64_t next_pow2(uint64_t x) {
     const uint64_t mask = (1ULL << lg) - 1ULL;                                           \
     memset(res, 0, sizeof(*res));                                                        \
     for (uint64_t i = 0; i < n; ++i) {                                                   \
+      if (keys[i] == (K)-1) continue; /* the reference's probe never ends on it */       \
       uint64_t idx = (uint64_t)orc_mult_shift_hash_##SUF(keys[i], lg);                   \
       for (uint64_t a = 0; a <= mask; ++a, idx = (idx + 1) & mask) {                     \
         if (table[idx].key == keys[i]) {                                                 \

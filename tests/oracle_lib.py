@@ -208,11 +208,14 @@ def radix_join(rk, rp, sk, sp, bits1, bits2=0, threads=4):
 
 def sort_merge_join(rk, rp, sk, sp):
     """Independent textbook check: count and wrapping sums of an equi-join
-    computed by sorting and grouping (numpy), not by hashing."""
+    computed by sorting and grouping (numpy), not by hashing.  The null key
+    (-1) never joins."""
     rk = np.asarray(rk, np.int64)
     sk = np.asarray(sk, np.int64)
     rp = np.asarray(rp, np.int64).view(np.uint64)
     sp = np.asarray(sp, np.int64).view(np.uint64)
+    rl, sl = rk != -1, sk != -1
+    rk, rp, sk, sp = rk[rl], rp[rl], sk[sl], sp[sl]
     ur, r_inv = np.unique(rk, return_inverse=True)
     r_cnt = np.bincount(r_inv).astype(np.uint64)
     with np.errstate(over="ignore"):

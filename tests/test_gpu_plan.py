@@ -1,0 +1,119 @@
+"""GPU parity of the batched radix-join execution method (fij_join_plan_*)
+against the oracle: same inputs, identical count and wrapping sums.
+
+Mirrors radix-join/tests/test_gpu_radix_join.rs (closed form N(N+1)/2 for
+PK/FK joins, :146-149) and adds duplicate keys, misses, negative payloads,
+single-pass plans and the BASELINE shapes at reduced size.
+"""
+import numpy as np
+import pytest
+
+import fij
+import oracle_lib as orc
+from gpu_helpers import to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def run_plan(rk, rp, sk, sp, bits1, bits2, **kw):
+    plan = fij.JoinPlan(rk.dtype, len(rk), len(sk), bits1, bits2, **kw)
+    got, timing = plan.execute(to_dev(rk), to_dev(rp), to_dev(sk), to_dev(sp))
+    assert fij.device_status() == 0
+    return got, timing
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("bits", [(9, 9), (6, 5), (4, 0), (8, 0), (1, 1)])
+def test_plan_pk_fk_closed_form(dtype, bits):
+    n = 1 << 20
+    rk = orc.gen_perm(n, 1, dtype, 1)
+    sk = orc.gen_perm(n, 2, dtype, 1)
+    rp = orc.gen_iota(n, 1, dtype)
+    sp = orc.gen_iota(n, 1, dtype)
+    got, _ = run_plan(rk, rp, sk, sp, *bits)
+    assert got == (n, n * (n + 1) // 2, n * (n + 1) // 2)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("bits", [(9, 9), (7, 3), (5, 0)])
+def test_plan_duplicates_misses(dtype, bits):
+    rk = orc.gen_uniform(300_000, 3, 1, 1, 100_000, dtype)
+    rp = orc.gen_uniform(300_000, 3, 2, -(1 << 20), 1 << 21, dtype)
+    sk = orc.gen_uniform(1_300_000, 3, 3, 1, 130_000, dtype)
+    sp = orc.gen_uniform(1_300_000, 3, 4, -(1 << 20), 1 << 21, dtype)
+    got, _ = run_plan(rk, rp, sk, sp, *bits)
+    assert got == orc.sort_merge_join(rk, rp, sk, sp)
+    assert got == orc.radix_join(rk, rp, sk, sp, bits[0], bits[1], threads=4)
+
+
+def test_plan_c1_shape():
+    """CPU-runnable shape: R 2^20 permutation of 1..2^20 (payload = key), S 2^24
+    uniform over R's keys (payload = row index), one pass of 8 bits."""
+    n_r, n_s = 1 << 20, 1 << 24
+    rk = orc.gen_perm(n_r, 42, np.int32, 1)
+    rp = rk.copy()
+    sk = orc.gen_uniform(n_s, 42, 1, 1, n_r, np.int32)
+    sp = orc.gen_iota(n_s, 0, np.int32)
+    got, _ = run_plan(rk, rp, sk, sp, 8, 0)
+    assert got == orc.radix_join(rk, rp, sk, sp, 8, 0, threads=8)
+    assert got[0] == n_s
+
+
+def test_plan_c2_shape_reduced():
+    """Full-scale shape at 1/16 size: R = S = 2^23 i64, 9 + 9 bits."""
+    n = 1 << 23
+    rk = orc.gen_perm(n, 42, np.int64, 1)
+    rp = rk.copy()
+    sk = orc.gen_uniform(n, 42, 1, 1, n, np.int64)
+    sp = orc.gen_iota(n, 0, np.int64)
+    got, _ = run_plan(rk, rp, sk, sp, 9, 9)
+    assert got == orc.radix_join(rk, rp, sk, sp, 9, 9, threads=8)
+
+
+def test_plan_repeat_and_chunks():
+    """A plan can be executed repeatedly; chunk count and block size change
+    nothing in the result."""
+    n = 1 << 19
+    rk = orc.gen_perm(n, 5, np.int64, 1)
+    sk = orc.gen_uniform(2 * n, 5, 1, 1, n, np.int64)
+    rp = orc.gen_uniform(n, 5, 2, 0, 1 << 30, np.int64)
+    sp = orc.gen_iota(2 * n, 0, np.int64)
+    want = orc.sort_merge_join(rk, rp, sk, sp)
+    for chunks, block in ((256, 1024), (512, 512), (100, 256)):
+        plan = fij.JoinPlan(np.int64, n, 2 * n, 9, 9, pass1_chunks=chunks, pass1_block=block)
+        dev = [to_dev(a) for a in (rk, rp, sk, sp)]
+        for _ in range(2):
+            got, _ = plan.execute(*dev)
+            assert got == want
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_null_keys_never_join(dtype):
+    rk = orc.gen_uniform(200_000, 4, 1, -3, 50_000, dtype)
+    sk = orc.gen_uniform(600_000, 4, 2, -3, 50_000, dtype)
+    rp = orc.gen_iota(200_000, 1, dtype)
+    sp = orc.gen_iota(600_000, 1, dtype)
+    want = orc.sort_merge_join(rk, rp, sk, sp)
+    got, _ = run_plan(rk, rp, sk, sp, 6, 5)
+    assert got == want
+    kt = fij.KeyType.I32 if dtype == np.int32 else fij.KeyType.I64
+    ht = fij.HashTable(1 << 19, kt)
+    hj = fij.HipHashJoin(fij.HashingScheme.LinearProbing, ht, (64, 256), (64, 256))
+    hj.build(to_dev(rk), to_dev(rp))
+    agg = fij.aggregates_array(64)
+    hj.probe_sum(to_dev(sk), to_dev(sp), None, agg)
+    fij.synchronize()
+    assert fij.sum_aggregates(agg) == want
+
+
+def test_plan_skewed_probe():
+    """One probe key carries a fifth of S: its sub-partition is probed by one
+    work item of the join."""
+    n = 1 << 18
+    rk = orc.gen_perm(n, 9, np.int64, 1)
+    rp = rk.copy()
+    sk = orc.gen_uniform(4 * n, 9, 1, 1, n, np.int64)
+    sk[: (4 * n) // 5] = 777
+    sp = orc.gen_iota(4 * n, 0, np.int64)
+    got, _ = run_plan(rk, rp, sk, sp, 9, 9)
+    assert got == orc.sort_merge_join(rk, rp, sk, sp)

@@ -254,6 +254,21 @@ def test_radix_join_brute_force_tiny():
     assert orc.sort_merge_join(rk, rp, sk, sp) == (cnt, sb, ss)
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_null_keys_never_join(dtype):
+    """-1 is the null key (gpu_common.h:109-123); inputs holding it join
+    exactly like inputs without those tuples."""
+    rk = orc.gen_uniform(20_000, 4, 1, -3, 500, dtype)   # includes -1
+    sk = orc.gen_uniform(60_000, 4, 2, -3, 500, dtype)
+    rp = orc.gen_iota(20_000, 1, dtype)
+    sp = orc.gen_iota(60_000, 1, dtype)
+    assert (rk == -1).any() and (sk == -1).any()
+    rl, sl = rk != -1, sk != -1
+    want = orc.sort_merge_join(rk[rl], rp[rl], sk[sl], sp[sl])
+    assert orc.radix_join(rk, rp, sk, sp, 4, 2) == want
+    assert orc.np_join_lp(rk, rp, sk, sp) == want
+
+
 def test_wrapping_sums():
     """Sums wrap modulo 2^64 like the reference's release-mode i64 sums."""
     n = 1000
