@@ -171,10 +171,9 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d, uint32_t
   }
   __syncthreads();
   block_exclusive_scan<uint32_t>(cnt, F2, wave_tmp);
-  // per-tile run starts of every pass-2 digit (meta[tile][F2] = n)
-  uint16_t* meta = s.meta + static_cast<uint64_t>(s.tile_start[p] + j) * (F2 + 1);
-  for (uint32_t i = tid; i < F2; i += BLOCK) meta[i] = static_cast<uint16_t>(cnt[i]);
-  if (tid == 0) meta[F2] = static_cast<uint16_t>(n);
+  // per-tile run bounds of every pass-2 digit: start | end << 16 (TILE2 <= 2^16)
+  uint32_t* meta = s.meta + static_cast<uint64_t>(s.tile_start[p] + j) * F2;
+  for (uint32_t i = tid; i < F2; i += BLOCK) meta[i] = cnt[i] | ((i + 1 < F2 ? cnt[i + 1] : n) << 16);
 #pragma unroll
   for (uint32_t k = 0; k < ITEMS; ++k) {
     const uint32_t x = lo + k * BLOCK + tid;
@@ -197,193 +196,319 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d, uint32_t
 
 constexpr uint32_t kMaxRuns = 64;
 
-// Run descriptor source for one side of a work item.
-struct RunList {
-  const void* base;   // Tuple array
-  uint32_t count;     // number of runs
-};
-
-// Loads run `r` (start index into the side's tuple array, length).
-__device__ __forceinline__ void run_of(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
-                                       uint32_t r, uint64_t& start, uint32_t& len) {
-  if (d.bits2) {
-    const uint32_t tile = s.tile_start[p] + r;
-    const uint16_t* m = s.meta + static_cast<uint64_t>(tile) * (d.F2 + 1);
-    const uint32_t b = m[q], e = m[q + 1];
-    start = s.pstart[p] + static_cast<uint64_t>(r) * d.tile2 + b;
-    len = e - b;
-  } else {
-    // single pass: run r is chunk r's piece of partition p
-    const uint32_t* ps = s.piece_start + static_cast<uint64_t>(p) * (d.G1 + 1);
-    start = s.off1[static_cast<uint64_t>(r) * d.F1 + p];
-    len = ps[r + 1] - ps[r];
-  }
+// Wave-uniform read of data that no launch in flight writes, through the
+// scalar data cache: it is counted apart from the vector loads, so waiting
+// for it does not wait for gathers already in flight.
+template <typename U>
+__device__ __forceinline__ U uniform_load(const U* ptr) {
+  return *(const __attribute__((address_space(4))) U*)(ptr);
 }
 
 __device__ __forceinline__ uint32_t run_count(const PipelineDevice& d, const PipelineSide& s, uint32_t p) {
-  return d.bits2 ? s.tile_start[p + 1] - s.tile_start[p] : d.G1;
+  return d.bits2 ? uniform_load(s.tile_start + p + 1) - uniform_load(s.tile_start + p) : d.G1;
 }
 
-template <typename K>
-struct WaveTable {
-  K* keys;
-  K* vals;
-  uint32_t cap;  // power of two
+// Loaded, not yet combined, descriptor of run r0 + lane of item (p, q).  With
+// two passes run r is pass-2 digit q's range inside tile r of pass-1
+// partition p: a = first tuple of the tile, b0 = its meta word (start | end
+// << 16).  With one pass run r is chunk r's piece of p: a = its first tuple,
+// [b0, b1) = its logical bounds.  Every lane loads from an in-bounds address
+// (lanes past the last run repeat it) and combining is left to the consumer:
+// no branch and no use follows the loads, so a prefetch does not make the
+// gathers issued before it wait.
+struct RawRun {
+  uint64_t a;
+  uint32_t b0, b1;
 };
 
-// Per-wave scratch: run starts / exclusive prefix of the current run batch.
+__device__ __forceinline__ RawRun raw_run(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
+                                          uint32_t r0, uint32_t nruns) {
+  RawRun x;
+  const uint32_t last = nruns ? nruns - 1 : 0;
+  const uint32_t r = min(r0 + lane_id(), last);
+  if (d.bits2) {
+    // nruns == 0 reads row tile_start[p] <= tiles: the meta has a spare row
+    const uint32_t tile = uniform_load(s.tile_start + p) + r;
+    x.a = uniform_load(s.pstart + p) + static_cast<uint64_t>(r) * d.tile2;
+    x.b0 = s.meta[static_cast<uint64_t>(tile) * d.F2 + q];
+    x.b1 = 0;
+  } else {
+    const uint32_t* ps = s.piece_start + static_cast<uint64_t>(p) * (d.G1 + 1);
+    x.a = s.off1[static_cast<uint64_t>(r) * d.F1 + p];
+    x.b0 = ps[r];
+    x.b1 = ps[r + 1];
+  }
+  return x;
+}
+
+// Lane-held descriptor of run r0 + lane (length 0 past the last run).
+struct LaneRun {
+  uint64_t start;
+  uint32_t len;
+};
+
+__device__ __forceinline__ LaneRun finish_run(const PipelineDevice& d, const RawRun& x, uint32_t r0,
+                                              uint32_t nruns) {
+  const bool live = r0 + lane_id() < nruns;
+  if (d.bits2) {
+    const uint32_t b = x.b0 & 0xffffu, e = x.b0 >> 16;
+    return LaneRun{x.a + b, live ? e - b : 0u};
+  }
+  return LaneRun{x.a, live ? x.b1 - x.b0 : 0u};
+}
+
+__device__ __forceinline__ LaneRun lane_run(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
+                                            uint32_t r0, uint32_t nruns) {
+  return finish_run(d, raw_run(d, s, p, q, r0, nruns), r0, nruns);
+}
+
+// Per-wave LDS copy of one side's run batch: run starts and the exclusive
+// prefix of the run lengths (prefix[kMaxRuns] = batch total).
 struct WaveRuns {
   uint64_t* start;   // [kMaxRuns]
   uint32_t* prefix;  // [kMaxRuns + 1]
 };
 
-// Fills the wave's run batch [r0, r0 + 64) of one side and returns the
-// number of tuples in it.
-__device__ uint32_t load_run_batch(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
-                                   uint32_t r0, uint32_t nruns, WaveRuns& wr) {
-  const uint32_t lane = lane_id();
-  const uint32_t r = r0 + lane;
-  uint64_t start = 0;
-  uint32_t len = 0;
-  if (r < nruns) run_of(d, s, p, q, r, start, len);
-  const uint32_t incl = wave_inclusive_scan(len);
-  wr.start[lane] = start;
-  wr.prefix[lane] = incl - len;
-  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-  if (lane == 0) wr.prefix[kMaxRuns] = total;
+__device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+}
+
+// Publishes a lane-held run batch to the wave's LDS; returns its tuple count.
+__device__ __forceinline__ uint32_t publish_runs(const LaneRun& x, const WaveRuns& wr) {
+  const uint32_t lane = lane_id();
+  const uint32_t incl = wave_inclusive_scan(x.len);
+  wr.start[lane] = x.start;
+  wr.prefix[lane] = incl - x.len;
+  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+  if (lane == 0) wr.prefix[kMaxRuns] = total;
   return total;
 }
 
-// Index of element e of the current batch: run with prefix[i] <= e < prefix[i+1].
+// Tuple index of element e of the published batch: the run i with
+// prefix[i] <= e < prefix[i + 1] (empty runs are skipped by taking the last
+// such i).
 __device__ __forceinline__ uint64_t batch_element(const WaveRuns& wr, uint32_t e) {
-  uint32_t l = 0, h = kMaxRuns;
-  while (h - l > 1) {
-    const uint32_t m = (l + h) / 2;
-    if (wr.prefix[m] <= e) l = m; else h = m;
-  }
+  // branch-free and fully unrolled: no loop between the gathers that follow,
+  // so their loads stay in flight together (prefix[0] = 0 <= e)
+  uint32_t l = 0;
+#pragma unroll
+  for (uint32_t step = kMaxRuns / 2; step > 0; step >>= 1)
+    l = wr.prefix[l + step] <= e ? l + step : l;
   return wr.start[l] + (e - wr.prefix[l]);
 }
 
-template <typename K, uint32_t CAP, uint32_t WAVES>
-__global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
-  using T = Tuple<K>;
-  constexpr uint32_t RCAP = CAP * 3 / 4;       // build tuples per table fill (load <= 3/4)
-  constexpr uint32_t BPL = 12;                 // build loads per lane per round
-  constexpr uint32_t SPL = 8;                  // probe loads per lane per round
-  extern __shared__ uint64_t lds_u64[];
-  const uint32_t w = threadIdx.x / 64;
+// Wave-private LDS hash table: linear probing from the multiply-shift hash
+// over CAP (power of two) slots; keys[] holds the null key in free slots.
+template <typename K, uint32_t CAP>
+struct WaveTable {
+  K* keys;
+  K* vals;
+  static constexpr uint32_t lg = 31u - __builtin_clz(CAP);
+
+  __device__ __forceinline__ void insert(K key, K val) const {
+    uint32_t h = mult_shift_hash<K>(key, lg);
+    for (;;) {
+      K prev;
+      if constexpr (sizeof(K) == 8)
+        prev = static_cast<K>(atomicCAS(reinterpret_cast<unsigned long long*>(&keys[h]),
+                                        static_cast<unsigned long long>(null_key<K>()),
+                                        static_cast<unsigned long long>(key)));
+      else
+        prev = static_cast<K>(atomicCAS(reinterpret_cast<unsigned int*>(&keys[h]),
+                                        static_cast<unsigned int>(null_key<K>()), static_cast<unsigned int>(key)));
+      if (prev == null_key<K>()) {
+        vals[h] = val;
+        return;
+      }
+      h = (h + 1) & (CAP - 1);
+    }
+  }
+
+  // Every build tuple with this key matches (probing continues to the first
+  // free slot, so duplicate build keys all count).
+  __device__ __forceinline__ void probe(K key, K pay, uint64_t& cnt, uint64_t& sb, uint64_t& sp) const {
+    uint32_t h = mult_shift_hash<K>(key, lg);
+    K kk;
+    while ((kk = keys[h]) != null_key<K>()) {
+      if (kk == key) {
+        cnt += 1;
+        sb += static_cast<uint64_t>(static_cast<int64_t>(vals[h]));
+        sp += static_cast<uint64_t>(static_cast<int64_t>(pay));
+      }
+      h = (h + 1) & (CAP - 1);
+    }
+  }
+
+  // The null key is all-ones for both key widths: 16-byte stores.
+  __device__ __forceinline__ void clear() const {
+    uint4* k4 = reinterpret_cast<uint4*>(keys);
+    constexpr uint32_t n4 = CAP * sizeof(K) / 16;
+    for (uint32_t i = lane_id(); i < n4; i += 64) k4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  }
+};
+
+template <typename K, uint32_t CAP>
+constexpr size_t join_wave_bytes() {
+  return (CAP * 2 * sizeof(K) + 2 * kMaxRuns * 8 + 2 * (kMaxRuns + 1) * 4 + 15) & ~size_t(15);
+}
+
+// Any item, any run count and size: run batches of 64 per side, table fills
+// of RCAP build tuples, every probe batch streamed against every fill.  The
+// table is clear on entry and on return.
+template <typename K, uint32_t CAP, uint32_t RCAP, uint32_t BPL, uint32_t SPL>
+__device__ void join_item_general(const PipelineDevice& d, uint32_t p, uint32_t q, const WaveTable<K, CAP>& tab,
+                                  const WaveRuns& rb, const WaveRuns& rp, const Tuple<K>* rel_b,
+                                  const Tuple<K>* rel_p, uint64_t& cnt, uint64_t& sb, uint64_t& sp) {
   const uint32_t lane = lane_id();
-  // per-wave LDS: keys[CAP], vals[CAP], run starts, run prefix
-  const size_t wave_bytes = CAP * 2 * sizeof(K) + kMaxRuns * 8 + (kMaxRuns + 1) * 4 + 4;
-  uint8_t* mine = reinterpret_cast<uint8_t*>(lds_u64) + w * ((wave_bytes + 15) & ~size_t(15));
-  WaveTable<K> tab{reinterpret_cast<K*>(mine), reinterpret_cast<K*>(mine) + CAP, CAP};
-  WaveRuns wr{reinterpret_cast<uint64_t*>(mine + CAP * 2 * sizeof(K)),
-              reinterpret_cast<uint32_t*>(mine + CAP * 2 * sizeof(K) + kMaxRuns * 8)};
-  const uint32_t lg = 31u - __builtin_clz(CAP);
-  for (uint32_t i = lane; i < CAP; i += 64) tab.keys[i] = null_key<K>();
-  const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
-  const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
-  const uint32_t F2 = d.bits2 ? d.F2 : 1;
-  const uint32_t items = d.F1 * F2;
-  uint64_t cnt = 0, sb = 0, sp = 0;
-  for (;;) {
-    uint32_t item = 0;
-    if (lane == 0) item = atomicAdd(d.work_counter, 1u);
-    item = __builtin_amdgcn_readfirstlane(item);
-    if (item >= items) break;
-    const uint32_t p = item / F2, q = item % F2;
-    const uint32_t nrb = run_count(d, d.build, p);
-    const uint32_t nrp = run_count(d, d.probe, p);
-    for (uint32_t rb0 = 0; rb0 < nrb; rb0 += kMaxRuns) {
-      const uint32_t tb = load_run_batch(d, d.build, p, q, rb0, nrb, wr);
-      for (uint32_t e0 = 0; e0 < tb; e0 += RCAP) {
-        // build: up to RCAP tuples per table fill, loaded BPL per lane at a
-        // time (independent loads), then inserted
-        const uint32_t e_end = min(tb, e0 + RCAP);
-        for (uint32_t e1 = e0; e1 < e_end; e1 += BPL * 64) {
+  const uint32_t nb = run_count(d, d.build, p);
+  const uint32_t np = run_count(d, d.probe, p);
+  for (uint32_t rb0 = 0; rb0 < nb; rb0 += kMaxRuns) {
+    const uint32_t tb = publish_runs(lane_run(d, d.build, p, q, rb0, nb), rb);
+    wave_lds_sync();
+    for (uint32_t e0 = 0; e0 < tb; e0 += RCAP) {
+      const uint32_t e_end = min(tb, e0 + RCAP);
+      for (uint32_t e1 = e0; e1 < e_end; e1 += BPL * 64) {
         K bk[BPL], bv[BPL];
 #pragma unroll
         for (uint32_t k = 0; k < BPL; ++k) {
           const uint32_t e = e1 + k * 64 + lane;
+          bk[k] = null_key<K>();
+          bv[k] = 0;
           if (e < e_end) {
-            const T t = load_tuple<K>(rel_b + batch_element(wr, e));
+            const Tuple<K> t = load_tuple<K>(rel_b + batch_element(rb, e));
             bk[k] = t.key;
             bv[k] = t.value;
-          } else {
-            bk[k] = null_key<K>();
           }
         }
 #pragma unroll
-        for (uint32_t k = 0; k < BPL; ++k) {
-          if (bk[k] != null_key<K>()) {
-            uint32_t h = mult_shift_hash<K>(bk[k], lg);
-            for (;;) {
-              K prev;
-              if constexpr (sizeof(K) == 8)
-                prev = static_cast<K>(atomicCAS(reinterpret_cast<unsigned long long*>(&tab.keys[h]),
-                                                static_cast<unsigned long long>(null_key<K>()),
-                                                static_cast<unsigned long long>(bk[k])));
-              else
-                prev = static_cast<K>(atomicCAS(reinterpret_cast<unsigned int*>(&tab.keys[h]),
-                                                static_cast<unsigned int>(null_key<K>()),
-                                                static_cast<unsigned int>(bk[k])));
-              if (prev == null_key<K>()) {
-                tab.vals[h] = bv[k];
-                break;
-              }
-              h = (h + 1) & (CAP - 1);
-            }
-          }
-        }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // probe every probe run of the item against this table fill
-        for (uint32_t rp0 = 0; rp0 < nrp; rp0 += kMaxRuns) {
-          // the run scratch is shared with the build side: save nothing, the
-          // build batch is reloaded below if another fill follows
-          const uint32_t tp = load_run_batch(d, d.probe, p, q, rp0, nrp, wr);
-          for (uint32_t f0 = 0; f0 < tp; f0 += SPL * 64) {
-            K pk[SPL], pv[SPL];
-#pragma unroll
-            for (uint32_t k = 0; k < SPL; ++k) {
-              const uint32_t e = f0 + k * 64 + lane;
-              if (e < tp) {
-                const T t = load_tuple<K>(rel_p + batch_element(wr, e));
-                pk[k] = t.key;
-                pv[k] = t.value;
-              } else {
-                pk[k] = null_key<K>();
-                pv[k] = 0;
-              }
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < SPL; ++k) {
-              if (pk[k] != null_key<K>()) {
-                uint32_t h = mult_shift_hash<K>(pk[k], lg);
-                K kk;
-                while ((kk = tab.keys[h]) != null_key<K>()) {
-                  if (kk == pk[k]) {
-                    cnt += 1;
-                    sb += static_cast<uint64_t>(static_cast<int64_t>(tab.vals[h]));
-                    sp += static_cast<uint64_t>(static_cast<int64_t>(pv[k]));
-                  }
-                  h = (h + 1) & (CAP - 1);
-                }
-              }
-            }
-          }
-        }
-        // clear the table and restore the build run batch for the next fill
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t i = lane; i < CAP; i += 64) tab.keys[i] = null_key<K>();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (e0 + RCAP < tb || nrp > kMaxRuns) load_run_batch(d, d.build, p, q, rb0, nrb, wr);
+        for (uint32_t k = 0; k < BPL; ++k)
+          if (bk[k] != null_key<K>()) tab.insert(bk[k], bv[k]);
       }
+      wave_lds_sync();
+      for (uint32_t rp0 = 0; rp0 < np; rp0 += kMaxRuns) {
+        const uint32_t tp = publish_runs(lane_run(d, d.probe, p, q, rp0, np), rp);
+        wave_lds_sync();
+        for (uint32_t f0 = 0; f0 < tp; f0 += SPL * 64) {
+          K pk[SPL], pv[SPL];
+#pragma unroll
+          for (uint32_t k = 0; k < SPL; ++k) {
+            const uint32_t e = f0 + k * 64 + lane;
+            pk[k] = null_key<K>();
+            pv[k] = 0;
+            if (e < tp) {
+              const Tuple<K> t = load_tuple<K>(rel_p + batch_element(rp, e));
+              pk[k] = t.key;
+              pv[k] = t.value;
+            }
+          }
+#pragma unroll
+          for (uint32_t k = 0; k < SPL; ++k)
+            if (pk[k] != null_key<K>()) tab.probe(pk[k], pv[k], cnt, sb, sp);
+        }
+        wave_lds_sync();
+      }
+      tab.clear();
+      wave_lds_sync();
     }
+  }
+}
+
+// One wave per work item (p, q).  Items are taken from an atomic counter in
+// tickets of up to 8 consecutive items.  The common item (at most 64 runs per
+// side, at most BPL * 64 build and SPL * 64 probe tuples) is one round trip:
+// both sides' tuples are loaded together, and the next item's run
+// descriptors are fetched while this item is built and probed.
+template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL>
+__global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
+  using T = Tuple<K>;
+  constexpr uint32_t RCAP = CAP * 3 / 4;  // build tuples per table fill (load factor <= 3/4)
+  constexpr uint32_t FAST_B = RCAP < BPL * 64 ? RCAP : BPL * 64;
+  extern __shared__ uint64_t lds_u64[];
+  const uint32_t w = threadIdx.x / 64;
+  const uint32_t lane = lane_id();
+  uint8_t* mine = reinterpret_cast<uint8_t*>(lds_u64) + w * join_wave_bytes<K, CAP>();
+  const WaveTable<K, CAP> tab{reinterpret_cast<K*>(mine), reinterpret_cast<K*>(mine) + CAP};
+  uint8_t* runs = mine + CAP * 2 * sizeof(K);
+  const WaveRuns rb{reinterpret_cast<uint64_t*>(runs),
+                    reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8)};
+  const WaveRuns rp{reinterpret_cast<uint64_t*>(runs + kMaxRuns * 8),
+                    reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8 + (kMaxRuns + 1) * 4)};
+  tab.clear();
+  wave_lds_sync();
+  const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
+  const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
+  const uint32_t F2 = d.bits2 ? d.F2 : 1;
+  const uint32_t items = d.F1 * F2;
+  const uint32_t per_wave = items / (gridDim.x * WAVES * 16);
+  const uint32_t tk = per_wave < 1 ? 1 : (per_wave > 8 ? 8 : per_wave);
+  uint32_t it = 0, it_end = 0;
+  auto next_item = [&]() -> uint32_t {
+    if (it == it_end) {
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(d.work_counter, tk);
+      t = __builtin_amdgcn_readfirstlane(t);
+      it = min(t, items);
+      it_end = min(t + tk, items);
+      if (it == it_end) return items;
+    }
+    return it++;
+  };
+  uint64_t cnt = 0, sb = 0, sp = 0;
+  uint32_t nb = 0, np = 0;
+  RawRun xb, xp;
+  // Run descriptors of item i (i < items).  Branch-free: issued behind the
+  // current item's gathers without making them wait.
+  auto fetch = [&](uint32_t i) {
+    const uint32_t p = i / F2, q = i % F2;
+    nb = run_count(d, d.build, p);
+    np = run_count(d, d.probe, p);
+    xb = raw_run(d, d.build, p, q, 0, nb);
+    xp = raw_run(d, d.probe, p, q, 0, np);
+  };
+  uint32_t cur = next_item();
+  if (cur < items) fetch(cur);
+  while (cur < items) {
+    const uint32_t tb = publish_runs(finish_run(d, xb, 0, nb), rb);
+    const uint32_t tp = publish_runs(finish_run(d, xp, 0, np), rp);
+    wave_lds_sync();
+    const bool fast = nb <= kMaxRuns && np <= kMaxRuns && tb <= FAST_B && tp <= SPL * 64;
+    const uint32_t nxt = next_item();
+    if ((tb == 0 && nb <= kMaxRuns) || (tp == 0 && np <= kMaxRuns)) {
+      // no build or no probe tuple (tb, tp count the first 64 runs only)
+      fetch(min(nxt, items - 1));
+    } else if (fast) {
+      // all gather indices first (LDS), then all gathers back to back and
+      // unconditionally (lanes past the end repeat the last tuple): an LDS
+      // read or a skipped load between two gathers would make the second
+      // wait for the first
+      uint64_t ib[BPL], ip[SPL];
+#pragma unroll
+      for (uint32_t k = 0; k < BPL; ++k) ib[k] = batch_element(rb, min(k * 64 + lane, tb - 1));
+#pragma unroll
+      for (uint32_t k = 0; k < SPL; ++k) ip[k] = batch_element(rp, min(k * 64 + lane, tp - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      T bt[BPL], pt[SPL];
+#pragma unroll
+      for (uint32_t k = 0; k < BPL; ++k) bt[k] = load_tuple<K>(rel_b + ib[k]);
+#pragma unroll
+      for (uint32_t k = 0; k < SPL; ++k) pt[k] = load_tuple<K>(rel_p + ip[k]);
+      fetch(min(nxt, items - 1));  // prefetch the next item's run descriptors
+#pragma unroll
+      for (uint32_t k = 0; k < BPL; ++k)
+        if (k * 64 + lane < tb && bt[k].key != null_key<K>()) tab.insert(bt[k].key, bt[k].value);
+      wave_lds_sync();
+#pragma unroll
+      for (uint32_t k = 0; k < SPL; ++k)
+        if (k * 64 + lane < tp && pt[k].key != null_key<K>()) tab.probe(pt[k].key, pt[k].value, cnt, sb, sp);
+      wave_lds_sync();
+      tab.clear();
+      wave_lds_sync();
+    } else {
+      join_item_general<K, CAP, RCAP, BPL, SPL>(d, cur / F2, cur % F2, tab, rb, rp, rel_b, rel_p, cnt, sb, sp);
+      fetch(min(nxt, items - 1));
+    }
+    cur = nxt;
   }
   // block reduction into aggregates[blockIdx.x]
   __shared__ uint64_t red[3][16];
@@ -411,8 +536,7 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
 
 template <typename K, uint32_t CAP, uint32_t WAVES>
 size_t join_lds_bytes() {
-  const size_t wave_bytes = CAP * 2 * sizeof(K) + kMaxRuns * 8 + (kMaxRuns + 1) * 4 + 4;
-  return WAVES * ((wave_bytes + 15) & ~size_t(15));
+  return WAVES * join_wave_bytes<K, CAP>();
 }
 
 template <typename K, uint32_t TILE2, uint32_t BLOCK>
@@ -464,7 +588,7 @@ size_t pipeline_sort2_lds(KeyType t, uint32_t F2, uint32_t G1) {
 
 template <typename K, uint32_t CAP, uint32_t WAVES>
 hipError_t launch_join_t(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
-  auto kern = join_kernel<K, CAP, WAVES>;
+  auto kern = join_kernel<K, CAP, WAVES, 10, 10>;
   const size_t lds = join_lds_bytes<K, CAP, WAVES>();
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));

@@ -25,7 +25,7 @@ struct PipelineSide {
   uint64_t* pstart;           // [F1 + 1] start of partition p in rel2
   uint32_t* tile_start;       // [F1 + 1] first pass-2 tile of partition p
   void* rel2;                 // Tuple[n]
-  uint16_t* meta;             // [tiles][F2 + 1] run starts per tile
+  uint32_t* meta;             // [tiles + 1][F2] per tile and pass-2 digit: run start | run end << 16
 };
 
 struct PipelineDevice {

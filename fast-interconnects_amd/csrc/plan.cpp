@@ -94,7 +94,7 @@ fij_status alloc_side(fij_join_plan* p, PipelineSide& s, uint64_t n, uint32_t ti
     uint8_t* rel2 = nullptr;
     if ((st = alloc(p, &rel2, n * tb))) return st;
     s.rel2 = rel2;
-    if ((st = alloc(p, &s.meta, static_cast<uint64_t>(tiles) * (F2 + 1)))) return st;
+    if ((st = alloc(p, &s.meta, (static_cast<uint64_t>(tiles) + 1) * F2))) return st;
   } else {
     s.rel2 = nullptr;
     s.meta = nullptr;
@@ -248,15 +248,19 @@ fij_status fij_join_plan_result(fij_join_plan* p, void* stream, fij_join_aggrega
     result->sum_probe = s;
   }
   if (timing) {
-    float a = 0, b = 0, c = 0, t = 0;
+    float a = 0, b = 0, c = 0, t = 0, c2 = 0, c3 = 0;
     PLAN_HIP(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
     PLAN_HIP(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
     PLAN_HIP(hipEventElapsedTime(&c, p->ev[2], p->ev[4]));
     PLAN_HIP(hipEventElapsedTime(&t, p->ev[0], p->ev[4]));
+    PLAN_HIP(hipEventElapsedTime(&c2, p->ev[2], p->ev[3]));
+    PLAN_HIP(hipEventElapsedTime(&c3, p->ev[3], p->ev[4]));
     timing->prefix_sum_ms = a;
     timing->partition_ms = b;
     timing->join_ms = c;
     timing->total_ms = t;
+    timing->pass2_ms = c2;
+    timing->build_probe_ms = c3;
   }
   return FIJ_OK;
 }
@@ -278,7 +282,7 @@ fij_status fij_join_plan_device_bytes(const fij_join_plan* p, uint64_t* bytes) {
     total += static_cast<uint64_t>(p->d.F1) * (p->d.G1 + 1) * 4 + p->d.F1 * 4 + (p->d.F1 + 1) * 12;
     if (p->d.bits2) total += s->n * tb;
   }
-  total += static_cast<uint64_t>(p->max_tiles[0] + p->max_tiles[1]) * (p->d.F2 + 1) * 2;
+  if (p->d.bits2) total += (static_cast<uint64_t>(p->max_tiles[0]) + p->max_tiles[1] + 2) * p->d.F2 * 4;
   *bytes = total;
   return FIJ_OK;
 }
