@@ -76,6 +76,22 @@ def cpu_baseline(log2n, bits1, bits2, threads):
     }
 
 
+def stream_copy_gbs(fij, nbytes, reps=5):
+    """HBM bandwidth of the library's plain copy kernel (bytes read + written
+    per second): what a streaming kernel reaches on this GPU."""
+    src = fij.DeviceArray(nbytes // 8, np.uint64)
+    dst = fij.DeviceArray(nbytes // 8, np.uint64)
+    src.fill(1)
+    fij.copy_kernel(dst, src)
+    e0, e1 = fij.Event(), fij.Event()
+    e0.record()
+    for _ in range(reps):
+        fij.copy_kernel(dst, src)
+    e1.record()
+    ms = e1.elapsed_ms(e0) / reps
+    return 2 * nbytes / (ms / 1000.0) / 1e9
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -157,6 +173,7 @@ def main():
         "phases_ms": med,
         "kernels": kernels,
         "pipeline_roofline_frac": b_alg / (gpu_ms / 1000.0) / (HBM_PEAK_GBS * 1e9),
+        "stream_copy_gbs": stream_copy_gbs(fij, 2 << 30),
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,

@@ -86,6 +86,23 @@ __global__ void fill_kernel(U* out, uint64_t n, U value) {
     out[i] = value;
 }
 
+// Plain device copy of 16-byte words, four independent loads in flight per
+// thread: the bandwidth reference the partition and join kernels are compared
+// with, and the known-size kernel that calibrates the HBM byte counters.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                   uint64_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], e = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = e;
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 uint32_t grid_for(uint64_t n) {
   uint64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -127,6 +144,12 @@ hipError_t launch_gen_iota(KeyType t, void* out, uint64_t n, int64_t base, hipSt
 
 hipError_t launch_fill_u64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream) {
   hipLaunchKernelGGL(fill_kernel<uint64_t>, dim3(grid_for(n)), dim3(256), 0, stream, out, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t words16, hipStream_t stream) {
+  hipLaunchKernelGGL(copy_kernel, dim3(grid_for(words16)), dim3(256), 0, stream, static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), words16);
   return hipGetLastError();
 }
 

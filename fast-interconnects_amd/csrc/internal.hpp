@@ -125,5 +125,6 @@ hipError_t launch_gen_uniform(KeyType t, void* out, uint64_t n, uint64_t seed, u
 hipError_t launch_gen_iota(KeyType t, void* out, uint64_t n, int64_t base, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream);
+hipError_t launch_copy(void* dst, const void* src, uint64_t words16, hipStream_t stream);
 
 }  // namespace fij

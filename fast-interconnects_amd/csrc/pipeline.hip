@@ -21,6 +21,7 @@
 // directly.  Work items of the join are taken from an atomic counter so that
 // skewed partitions do not serialise a fixed wave.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(1024) void plan_tiles_kernel(PipelineDevice d) {
     s.tile_start[F1] = static_cast<uint32_t>(total_tiles);
   }
   if (rel == 0) {
-    if (threadIdx.x == 0) *d.work_counter = 0;
+    if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
     for (uint32_t i = threadIdx.x; i < d.agg_len; i += blockDim.x) {
       d.aggregates[i].count = 0;
       d.aggregates[i].sum_build = 0;
@@ -299,25 +300,27 @@ __device__ __forceinline__ uint64_t batch_element(const WaveRuns& wr, uint32_t e
 }
 
 // Wave-private LDS hash table: linear probing from the multiply-shift hash
-// over CAP (power of two) slots; keys[] holds the null key in free slots.
-template <typename K, uint32_t CAP>
+// over CAP (power of two) slots.  KT is the stored key type; a free slot holds
+// all ones.  Values are stored beside the keys.
+template <typename KT, typename V, uint32_t CAP>
 struct WaveTable {
-  K* keys;
-  K* vals;
+  KT* keys;
+  V* vals;
   static constexpr uint32_t lg = 31u - __builtin_clz(CAP);
+  static constexpr KT kEmpty = static_cast<KT>(~static_cast<KT>(0));
 
-  __device__ __forceinline__ void insert(K key, K val) const {
-    uint32_t h = mult_shift_hash<K>(key, lg);
+  __device__ __forceinline__ void insert(KT key, V val) const {
+    uint32_t h = mult_shift_hash<KT>(key, lg);
     for (;;) {
-      K prev;
-      if constexpr (sizeof(K) == 8)
-        prev = static_cast<K>(atomicCAS(reinterpret_cast<unsigned long long*>(&keys[h]),
-                                        static_cast<unsigned long long>(null_key<K>()),
-                                        static_cast<unsigned long long>(key)));
+      KT prev;
+      if constexpr (sizeof(KT) == 8)
+        prev = static_cast<KT>(atomicCAS(reinterpret_cast<unsigned long long*>(&keys[h]),
+                                         static_cast<unsigned long long>(kEmpty),
+                                         static_cast<unsigned long long>(key)));
       else
-        prev = static_cast<K>(atomicCAS(reinterpret_cast<unsigned int*>(&keys[h]),
-                                        static_cast<unsigned int>(null_key<K>()), static_cast<unsigned int>(key)));
-      if (prev == null_key<K>()) {
+        prev = static_cast<KT>(atomicCAS(reinterpret_cast<unsigned int*>(&keys[h]), static_cast<unsigned int>(kEmpty),
+                                         static_cast<unsigned int>(key)));
+      if (prev == kEmpty) {
         vals[h] = val;
         return;
       }
@@ -327,10 +330,10 @@ struct WaveTable {
 
   // Every build tuple with this key matches (probing continues to the first
   // free slot, so duplicate build keys all count).
-  __device__ __forceinline__ void probe(K key, K pay, uint64_t& cnt, uint64_t& sb, uint64_t& sp) const {
-    uint32_t h = mult_shift_hash<K>(key, lg);
-    K kk;
-    while ((kk = keys[h]) != null_key<K>()) {
+  __device__ __forceinline__ void probe(KT key, V pay, uint64_t& cnt, uint64_t& sb, uint64_t& sp) const {
+    uint32_t h = mult_shift_hash<KT>(key, lg);
+    KT kk;
+    while ((kk = keys[h]) != kEmpty) {
       if (kk == key) {
         cnt += 1;
         sb += static_cast<uint64_t>(static_cast<int64_t>(vals[h]));
@@ -340,10 +343,10 @@ struct WaveTable {
     }
   }
 
-  // The null key is all-ones for both key widths: 16-byte stores.
+  // 16-byte stores of all ones.
   __device__ __forceinline__ void clear() const {
     uint4* k4 = reinterpret_cast<uint4*>(keys);
-    constexpr uint32_t n4 = CAP * sizeof(K) / 16;
+    constexpr uint32_t n4 = CAP * sizeof(KT) / 16;
     for (uint32_t i = lane_id(); i < n4; i += 64) k4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
   }
 };
@@ -355,11 +358,14 @@ constexpr size_t join_wave_bytes() {
 
 // Any item, any run count and size: run batches of 64 per side, table fills
 // of RCAP build tuples, every probe batch streamed against every fill.  The
-// table is clear on entry and on return.
+// table is clear on entry and on return.  bt / pt are register scratch (the
+// caller's idle pipeline buffers).
 template <typename K, uint32_t CAP, uint32_t RCAP, uint32_t BPL, uint32_t SPL>
-__device__ void join_item_general(const PipelineDevice& d, uint32_t p, uint32_t q, const WaveTable<K, CAP>& tab,
-                                  const WaveRuns& rb, const WaveRuns& rp, const Tuple<K>* rel_b,
-                                  const Tuple<K>* rel_p, uint64_t& cnt, uint64_t& sb, uint64_t& sp) {
+__device__ __forceinline__ void join_item_general(const PipelineDevice& d, uint32_t p, uint32_t q, K* table_keys,
+                                                  const WaveRuns& rb, const WaveRuns& rp, const Tuple<K>* rel_b,
+                                                  const Tuple<K>* rel_p, Tuple<K> (&bt)[BPL], Tuple<K> (&pt)[SPL],
+                                                  uint64_t& cnt, uint64_t& sb, uint64_t& sp) {
+  const WaveTable<K, K, CAP> tab{table_keys, table_keys + CAP};
   const uint32_t lane = lane_id();
   const uint32_t nb = run_count(d, d.build, p);
   const uint32_t np = run_count(d, d.probe, p);
@@ -369,42 +375,30 @@ __device__ void join_item_general(const PipelineDevice& d, uint32_t p, uint32_t 
     for (uint32_t e0 = 0; e0 < tb; e0 += RCAP) {
       const uint32_t e_end = min(tb, e0 + RCAP);
       for (uint32_t e1 = e0; e1 < e_end; e1 += BPL * 64) {
-        K bk[BPL], bv[BPL];
 #pragma unroll
         for (uint32_t k = 0; k < BPL; ++k) {
           const uint32_t e = e1 + k * 64 + lane;
-          bk[k] = null_key<K>();
-          bv[k] = 0;
-          if (e < e_end) {
-            const Tuple<K> t = load_tuple<K>(rel_b + batch_element(rb, e));
-            bk[k] = t.key;
-            bv[k] = t.value;
-          }
+          bt[k].key = null_key<K>();
+          if (e < e_end) bt[k] = load_tuple<K>(rel_b + batch_element(rb, e));
         }
 #pragma unroll
         for (uint32_t k = 0; k < BPL; ++k)
-          if (bk[k] != null_key<K>()) tab.insert(bk[k], bv[k]);
+          if (bt[k].key != null_key<K>()) tab.insert(bt[k].key, bt[k].value);
       }
       wave_lds_sync();
       for (uint32_t rp0 = 0; rp0 < np; rp0 += kMaxRuns) {
         const uint32_t tp = publish_runs(lane_run(d, d.probe, p, q, rp0, np), rp);
         wave_lds_sync();
         for (uint32_t f0 = 0; f0 < tp; f0 += SPL * 64) {
-          K pk[SPL], pv[SPL];
 #pragma unroll
           for (uint32_t k = 0; k < SPL; ++k) {
             const uint32_t e = f0 + k * 64 + lane;
-            pk[k] = null_key<K>();
-            pv[k] = 0;
-            if (e < tp) {
-              const Tuple<K> t = load_tuple<K>(rel_p + batch_element(rp, e));
-              pk[k] = t.key;
-              pv[k] = t.value;
-            }
+            pt[k].key = null_key<K>();
+            if (e < tp) pt[k] = load_tuple<K>(rel_p + batch_element(rp, e));
           }
 #pragma unroll
           for (uint32_t k = 0; k < SPL; ++k)
-            if (pk[k] != null_key<K>()) tab.probe(pk[k], pv[k], cnt, sb, sp);
+            if (pt[k].key != null_key<K>()) tab.probe(pt[k].key, pt[k].value, cnt, sb, sp);
         }
         wave_lds_sync();
       }
@@ -414,45 +408,198 @@ __device__ void join_item_general(const PipelineDevice& d, uint32_t p, uint32_t 
   }
 }
 
-// One wave per work item (p, q).  Items are taken from an atomic counter in
-// tickets of up to 8 consecutive items.  The common item (at most 64 runs per
-// side, at most BPL * 64 build and SPL * 64 probe tuples) is one round trip:
-// both sides' tuples are loaded together, and the next item's run
-// descriptors are fetched while this item is built and probed.
+// Build and probe of one item whose tuples are in registers (at most BPL * 64
+// build and SPL * 64 probe tuples; lanes past tb / tp hold copies and are
+// ignored).  All keys of the item share their low `shift` bits (its radix
+// digits), so two keys are equal iff their values >> shift are.  When every
+// build key's upper part fits 32 bits (below all ones), the table stores
+// that narrow key: half the LDS traffic of 64-bit keys.
+template <typename K, uint32_t CAP, uint32_t BPL, uint32_t SPL, bool NARROW>
+__device__ __forceinline__ void join_in_registers(K* table_keys, const Tuple<K> (&bt)[BPL],
+                                                  const Tuple<K> (&pt)[SPL], uint32_t tb, uint32_t tp,
+                                                  uint32_t shift, uint64_t& cnt, uint64_t& sb, uint64_t& sp) {
+  const uint32_t lane = lane_id();
+  bool narrow = NARROW;
+  if constexpr (NARROW && sizeof(K) == 8) {
+    bool wide = false;
+#pragma unroll
+    for (uint32_t k = 0; k < BPL; ++k)
+      wide |= k * 64 + lane < tb && bt[k].key != null_key<K>() &&
+              (static_cast<uint64_t>(bt[k].key) >> shift) >= 0xffffffffull;
+    narrow = __builtin_amdgcn_ballot_w64(wide) == 0;
+  }
+  if (narrow) {
+    const WaveTable<uint32_t, K, CAP> tab{reinterpret_cast<uint32_t*>(table_keys), table_keys + CAP};
+#pragma unroll
+    for (uint32_t k = 0; k < BPL; ++k)
+      if (k * 64 + lane < tb && bt[k].key != null_key<K>())
+        tab.insert(static_cast<uint32_t>(static_cast<uint64_t>(static_cast<std::make_unsigned_t<K>>(bt[k].key)) >> shift),
+                   bt[k].value);
+    wave_lds_sync();
+#pragma unroll
+    for (uint32_t k = 0; k < SPL; ++k) {
+      const uint64_t hi = static_cast<uint64_t>(static_cast<std::make_unsigned_t<K>>(pt[k].key)) >> shift;
+      // a probe key whose upper part does not fit matches no build key
+      if (k * 64 + lane < tp && pt[k].key != null_key<K>() && hi < 0xffffffffull)
+        tab.probe(static_cast<uint32_t>(hi), pt[k].value, cnt, sb, sp);
+    }
+    wave_lds_sync();
+    tab.clear();
+  } else {
+    const WaveTable<K, K, CAP> tab{table_keys, table_keys + CAP};
+#pragma unroll
+    for (uint32_t k = 0; k < BPL; ++k)
+      if (k * 64 + lane < tb && bt[k].key != null_key<K>()) tab.insert(bt[k].key, bt[k].value);
+    wave_lds_sync();
+#pragma unroll
+    for (uint32_t k = 0; k < SPL; ++k)
+      if (k * 64 + lane < tp && pt[k].key != null_key<K>()) tab.probe(pt[k].key, pt[k].value, cnt, sb, sp);
+    wave_lds_sync();
+    tab.clear();
+  }
+  wave_lds_sync();
+}
+
+// Per-block {count, SUM(build.value), SUM(probe.value)} added to
+// aggregates[blockIdx.x].
+template <uint32_t WAVES>
+__device__ __forceinline__ void join_block_aggregate(const PipelineDevice& d, uint64_t cnt, uint64_t sb, uint64_t sp) {
+  __shared__ uint64_t red[3][WAVES];
+  const uint32_t w = threadIdx.x / 64;
+  const uint64_t wc = wave_broadcast_last(wave_inclusive_scan(cnt));
+  const uint64_t wb = wave_broadcast_last(wave_inclusive_scan(sb));
+  const uint64_t wp = wave_broadcast_last(wave_inclusive_scan(sp));
+  if (lane_id() == 0) {
+    red[0][w] = wc;
+    red[1][w] = wb;
+    red[2][w] = wp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t a = 0, b = 0, c = 0;
+    for (uint32_t i = 0; i < WAVES; ++i) {
+      a += red[0][i];
+      b += red[1][i];
+      c += red[2][i];
+    }
+    d.aggregates[blockIdx.x].count += a;
+    d.aggregates[blockIdx.x].sum_build += b;
+    d.aggregates[blockIdx.x].sum_probe += c;
+  }
+}
+
+// The per-wave LDS layout shared by both join kernels: the table (keys, then
+// values) and the two sides' run batches.
+template <typename K, uint32_t CAP>
+struct WaveLds {
+  K* table_keys;
+  WaveRuns rb, rp;
+  __device__ __forceinline__ explicit WaveLds(uint64_t* lds) {
+    uint8_t* mine = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x / 64) * join_wave_bytes<K, CAP>();
+    table_keys = reinterpret_cast<K*>(mine);
+    uint8_t* runs = mine + CAP * 2 * sizeof(K);
+    rb = WaveRuns{reinterpret_cast<uint64_t*>(runs), reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8)};
+    rp = WaveRuns{reinterpret_cast<uint64_t*>(runs + kMaxRuns * 8),
+                  reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8 + (kMaxRuns + 1) * 4)};
+  }
+};
+
+// The items the register-resident join deferred (more than 64 runs or more
+// tuples than its registers hold on a side): it sets overflow[item] = 1.
+// Waves take tickets of 64 items, join every flagged one (one wave per item)
+// and clear its flag, so the flags are all zero again for the next join.
 template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL>
+__global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice d) {
+  using T = Tuple<K>;
+  constexpr uint32_t RCAP = CAP * 3 / 4;
+  extern __shared__ uint64_t lds_u64[];
+  const WaveLds<K, CAP> l(lds_u64);
+  const uint32_t lane = lane_id();
+  const uint32_t F2 = d.bits2 ? d.F2 : 1;
+  const uint32_t items = d.F1 * F2;
+  const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
+  const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
+  uint64_t cnt = 0, sb = 0, sp = 0;
+  bool cleared = false;
+  T bt[BPL], pt[SPL];
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(d.work_counter + kOverflowTicket, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (static_cast<uint64_t>(t) * 64 >= items) break;
+    const uint32_t i = t * 64 + lane;
+    const bool flagged = i < items && d.overflow[i] != 0;
+    uint64_t m = __builtin_amdgcn_ballot_w64(flagged);
+    if (flagged) d.overflow[i] = 0;
+    if (m && !cleared) {
+      WaveTable<K, K, CAP>{l.table_keys, l.table_keys + CAP}.clear();
+      wave_lds_sync();
+      cleared = true;
+    }
+    while (m) {
+      const uint32_t item = t * 64 + static_cast<uint32_t>(__builtin_ctzll(m));
+      m &= m - 1;
+      join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, l.table_keys, l.rb, l.rp, rel_b, rel_p, bt,
+                                                pt, cnt, sb, sp);
+    }
+  }
+  join_block_aggregate<WAVES>(d, cnt, sb, sp);
+}
+
+// One wave per work item (p, q).  Items are handed out in tickets of up to 8
+// consecutive items.  With XCD = true, work group g draws first from the
+// items of pass-1 partitions p = g mod 8 (its XCD's share: neighbouring runs
+// of consecutive q share cache lines, which then stay in that XCD's L2) and
+// then from the other shares.  The common item (at most 64 runs per side, at
+// most BPL * 64 build and SPL * 64 probe tuples) is software-pipelined: its
+// gathers are issued, then the previous item is built and probed from
+// registers while they are in flight, and the next item's run descriptors
+// are fetched behind them.
+template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL, bool NARROW, bool XCD>
 __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
   using T = Tuple<K>;
   constexpr uint32_t RCAP = CAP * 3 / 4;  // build tuples per table fill (load factor <= 3/4)
   constexpr uint32_t FAST_B = RCAP < BPL * 64 ? RCAP : BPL * 64;
+  static_assert(WAVES <= 16, "aggregate scratch");
   extern __shared__ uint64_t lds_u64[];
-  const uint32_t w = threadIdx.x / 64;
+  const WaveLds<K, CAP> l(lds_u64);
+  K* const table_keys = l.table_keys;
+  const WaveRuns& rb = l.rb;
+  const WaveRuns& rp = l.rp;
   const uint32_t lane = lane_id();
-  uint8_t* mine = reinterpret_cast<uint8_t*>(lds_u64) + w * join_wave_bytes<K, CAP>();
-  const WaveTable<K, CAP> tab{reinterpret_cast<K*>(mine), reinterpret_cast<K*>(mine) + CAP};
-  uint8_t* runs = mine + CAP * 2 * sizeof(K);
-  const WaveRuns rb{reinterpret_cast<uint64_t*>(runs),
-                    reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8)};
-  const WaveRuns rp{reinterpret_cast<uint64_t*>(runs + kMaxRuns * 8),
-                    reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8 + (kMaxRuns + 1) * 4)};
-  tab.clear();
+  WaveTable<K, K, CAP>{table_keys, table_keys + CAP}.clear();
   wave_lds_sync();
   const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
   const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
+  const uint32_t F1 = d.F1;
   const uint32_t F2 = d.bits2 ? d.F2 : 1;
-  const uint32_t items = d.F1 * F2;
+  const uint32_t shift = d.bits1 + d.bits2;
+  const uint32_t items = F1 * F2;
+  const T* const dummy = reinterpret_cast<const T*>(d.aggregates);  // 16 readable bytes, value unused
   const uint32_t per_wave = items / (gridDim.x * WAVES * 16);
   const uint32_t tk = per_wave < 1 ? 1 : (per_wave > 8 ? 8 : per_wave);
-  uint32_t it = 0, it_end = 0;
+  // ticket state: group share being drawn from, position in it
+  const uint32_t g0 = XCD ? blockIdx.x % kJoinCounters : 0;
+  uint32_t gi = 0, grp = g0, it = 0, it_end = 0;
   auto next_item = [&]() -> uint32_t {
-    if (it == it_end) {
+    while (it == it_end) {
+      if (gi >= (XCD ? kJoinCounters : 1u)) return items;
+      const uint32_t g = XCD ? (g0 + gi) % kJoinCounters : 0;
+      const uint32_t len = XCD ? (g < F1 ? (F1 - g + kJoinCounters - 1) / kJoinCounters * F2 : 0) : items;
       uint32_t t = 0;
-      if (lane == 0) t = atomicAdd(d.work_counter, tk);
+      if (lane == 0 && len) t = atomicAdd(d.work_counter + g * kJoinCounterStride, tk);
       t = __builtin_amdgcn_readfirstlane(t);
-      it = min(t, items);
-      it_end = min(t + tk, items);
-      if (it == it_end) return items;
+      if (len && t < len) {
+        grp = g;
+        it = t;
+        it_end = min(t + tk, len);
+      } else {
+        ++gi;
+      }
     }
-    return it++;
+    const uint32_t j = it++;
+    if (!XCD) return j;
+    return (grp + kJoinCounters * (j / F2)) * F2 + j % F2;
   };
   uint64_t cnt = 0, sb = 0, sp = 0;
   uint32_t nb = 0, np = 0;
@@ -468,70 +615,70 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
   };
   uint32_t cur = next_item();
   if (cur < items) fetch(cur);
-  while (cur < items) {
-    const uint32_t tb = publish_runs(finish_run(d, xb, 0, nb), rb);
-    const uint32_t tp = publish_runs(finish_run(d, xp, 0, np), rp);
-    wave_lds_sync();
-    const bool fast = nb <= kMaxRuns && np <= kMaxRuns && tb <= FAST_B && tp <= SPL * 64;
-    const uint32_t nxt = next_item();
-    if ((tb == 0 && nb <= kMaxRuns) || (tp == 0 && np <= kMaxRuns)) {
-      // no build or no probe tuple (tb, tp count the first 64 runs only)
-      fetch(min(nxt, items - 1));
-    } else if (fast) {
-      // all gather indices first (LDS), then all gathers back to back and
-      // unconditionally (lanes past the end repeat the last tuple): an LDS
-      // read or a skipped load between two gathers would make the second
-      // wait for the first
-      uint64_t ib[BPL], ip[SPL];
+  bool pend = false;  // an item's tuples are in flight in the work buffer
+  uint32_t pend_tb = 0, pend_tp = 0;
+  // One pipeline step: stage item `cur` into (sbuf, spbuf) and finish the
+  // pending item held in (wbuf, wpbuf).  Returns false when no items are
+  // left (the pending item, if any, is finished first).  The build/probe has
+  // a single call site, which keeps the register allocation of the two
+  // buffers simple.
+  auto step = [&](T (&sbuf)[BPL], T (&spbuf)[SPL], T (&wbuf)[BPL], T (&wpbuf)[SPL]) -> bool {
+    const bool live = cur < items;
+    bool staged = false;
+    uint32_t tb = 0, tp = 0, nxt = items;
+    uint64_t ib[BPL], ip[SPL];
+    const T* gb = dummy;
+    const T* gp = dummy;
 #pragma unroll
-      for (uint32_t k = 0; k < BPL; ++k) ib[k] = batch_element(rb, min(k * 64 + lane, tb - 1));
+    for (uint32_t k = 0; k < BPL; ++k) ib[k] = 0;
 #pragma unroll
-      for (uint32_t k = 0; k < SPL; ++k) ip[k] = batch_element(rp, min(k * 64 + lane, tp - 1));
-      __builtin_amdgcn_sched_barrier(0);
-      T bt[BPL], pt[SPL];
-#pragma unroll
-      for (uint32_t k = 0; k < BPL; ++k) bt[k] = load_tuple<K>(rel_b + ib[k]);
-#pragma unroll
-      for (uint32_t k = 0; k < SPL; ++k) pt[k] = load_tuple<K>(rel_p + ip[k]);
-      fetch(min(nxt, items - 1));  // prefetch the next item's run descriptors
-#pragma unroll
-      for (uint32_t k = 0; k < BPL; ++k)
-        if (k * 64 + lane < tb && bt[k].key != null_key<K>()) tab.insert(bt[k].key, bt[k].value);
+    for (uint32_t k = 0; k < SPL; ++k) ip[k] = 0;
+    if (live) {
+      tb = publish_runs(finish_run(d, xb, 0, nb), rb);
+      tp = publish_runs(finish_run(d, xp, 0, np), rp);
       wave_lds_sync();
+      nxt = next_item();
+      const bool empty = (tb == 0 && nb <= kMaxRuns) || (tp == 0 && np <= kMaxRuns);  // tb, tp: first 64 runs
+      const bool fast = nb <= kMaxRuns && np <= kMaxRuns && tb <= FAST_B && tp <= SPL * 64;
+      if (!empty && fast) {
+        // gather indices from LDS (lanes past the end repeat the last tuple)
 #pragma unroll
-      for (uint32_t k = 0; k < SPL; ++k)
-        if (k * 64 + lane < tp && pt[k].key != null_key<K>()) tab.probe(pt[k].key, pt[k].value, cnt, sb, sp);
-      wave_lds_sync();
-      tab.clear();
-      wave_lds_sync();
-    } else {
-      join_item_general<K, CAP, RCAP, BPL, SPL>(d, cur / F2, cur % F2, tab, rb, rp, rel_b, rel_p, cnt, sb, sp);
-      fetch(min(nxt, items - 1));
+        for (uint32_t k = 0; k < BPL; ++k) ib[k] = batch_element(rb, min(k * 64 + lane, tb - 1));
+#pragma unroll
+        for (uint32_t k = 0; k < SPL; ++k) ip[k] = batch_element(rp, min(k * 64 + lane, tp - 1));
+        gb = rel_b;
+        gp = rel_p;
+        staged = true;
+      } else if (!empty && lane == 0) {
+        // does not fit the registers: deferred to join_general_kernel (an
+        // empty item has no match)
+        d.overflow[cur] = 1;
+      }
     }
+    // Every step issues the same gathers back to back (a step that stages
+    // nothing reads a fixed in-bounds word), so the wait before the pending
+    // item's build counts exactly this step's loads on every path: an LDS
+    // read, a branch or a skipped load between gathers would make the build
+    // wait for this step's gathers too.
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (uint32_t k = 0; k < BPL; ++k) sbuf[k] = load_tuple<K>(gb + ib[k]);
+#pragma unroll
+    for (uint32_t k = 0; k < SPL; ++k) spbuf[k] = load_tuple<K>(gp + ip[k]);
+    if (live) fetch(min(nxt, items - 1));
+    if (pend) join_in_registers<K, CAP, BPL, SPL, NARROW>(table_keys, wbuf, wpbuf, pend_tb, pend_tp, shift, cnt, sb, sp);
+    pend = staged;
+    pend_tb = tb;
+    pend_tp = tp;
     cur = nxt;
+    return live;
+  };
+  T ab[BPL], ap[SPL], bb[BPL], bp[SPL];
+  for (;;) {
+    if (!step(ab, ap, bb, bp)) break;
+    if (!step(bb, bp, ab, ap)) break;
   }
-  // block reduction into aggregates[blockIdx.x]
-  __shared__ uint64_t red[3][16];
-  const uint64_t wc = wave_broadcast_last(wave_inclusive_scan(cnt));
-  const uint64_t wb = wave_broadcast_last(wave_inclusive_scan(sb));
-  const uint64_t wp = wave_broadcast_last(wave_inclusive_scan(sp));
-  if (lane == 0) {
-    red[0][w] = wc;
-    red[1][w] = wb;
-    red[2][w] = wp;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t a = 0, bb = 0, c = 0;
-    for (uint32_t i = 0; i < WAVES; ++i) {
-      a += red[0][i];
-      bb += red[1][i];
-      c += red[2][i];
-    }
-    d.aggregates[blockIdx.x].count += a;
-    d.aggregates[blockIdx.x].sum_build += bb;
-    d.aggregates[blockIdx.x].sum_probe += c;
-  }
+  join_block_aggregate<WAVES>(d, cnt, sb, sp);
 }
 
 template <typename K, uint32_t CAP, uint32_t WAVES>
@@ -588,11 +735,18 @@ size_t pipeline_sort2_lds(KeyType t, uint32_t F2, uint32_t G1) {
 
 template <typename K, uint32_t CAP, uint32_t WAVES>
 hipError_t launch_join_t(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
-  auto kern = join_kernel<K, CAP, WAVES, 10, 10>;
+  constexpr uint32_t PL = sizeof(K) == 8 ? 9 : 10;  // tuples per lane and side (register budget)
   const size_t lds = join_lds_bytes<K, CAP, WAVES>();
+  auto kern = join_kernel<K, CAP, WAVES, PL, PL, true, true>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  auto gen = join_general_kernel<K, CAP, WAVES, 12, 8>;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(gen), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds));
+  hipLaunchKernelGGL(gen, dim3(grid), dim3(WAVES * 64), lds, stream, d);
   return hipGetLastError();
 }
 

@@ -28,6 +28,14 @@ struct PipelineSide {
   uint32_t* meta;             // [tiles + 1][F2] per tile and pass-2 digit: run start | run end << 16
 };
 
+// Join work counters, each on its own 128-byte line: one per XCD (work
+// groups are dispatched to the 8 XCDs round-robin), then the ticket counter
+// of the kernel that joins the deferred items.
+constexpr uint32_t kJoinCounters = 8;
+constexpr uint32_t kJoinCounterStride = 32;
+constexpr uint32_t kOverflowTicket = kJoinCounters * kJoinCounterStride;
+constexpr uint32_t kJoinCounterWords = (kJoinCounters + 1) * kJoinCounterStride;
+
 struct PipelineDevice {
   PipelineSide build;
   PipelineSide probe;
@@ -35,7 +43,8 @@ struct PipelineDevice {
   uint32_t bits1, bits2;
   uint32_t pad;
   uint32_t tile2;
-  uint32_t* work_counter;
+  uint32_t* work_counter;     // [kJoinCounterWords]
+  uint8_t* overflow;          // [F1 * F2] 1 = item deferred to the general join (zero between joins)
   JoinAggregate* aggregates;  // [agg_len]
   uint32_t agg_len;
 };

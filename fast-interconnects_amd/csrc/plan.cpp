@@ -161,7 +161,13 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
     st = plan_fail(FIJ_UNSUPPORTED, "pass-2 tile does not fit LDS for this fanout / chunk count");
   if (st == FIJ_OK) st = alloc_side(p, p->d.build, cfg->build_len, p->max_tiles[0]);
   if (st == FIJ_OK) st = alloc_side(p, p->d.probe, cfg->probe_len, p->max_tiles[1]);
-  if (st == FIJ_OK) st = alloc(p, &p->d.work_counter, 1);
+  if (st == FIJ_OK) st = alloc(p, &p->d.work_counter, kJoinCounterWords);
+  const uint64_t items = static_cast<uint64_t>(p->d.F1) * (p->d.bits2 ? p->d.F2 : 1);
+  if (st == FIJ_OK) st = alloc(p, &p->d.overflow, items);
+  // the deferred-item flags start at zero (complete before any stream uses
+  // them); every join leaves them at zero
+  if (st == FIJ_OK && (hipMemset(p->d.overflow, 0, items) != hipSuccess || hipDeviceSynchronize() != hipSuccess))
+    st = plan_fail(FIJ_DEVICE_ERROR, "clearing the deferred-item flags failed");
   if (st == FIJ_OK) st = alloc(p, &p->d.aggregates, p->join_grid);
   for (int i = 0; i < 5 && st == FIJ_OK; ++i)
     if (hipEventCreate(&p->ev[i]) != hipSuccess) st = plan_fail(FIJ_DEVICE_ERROR, "hipEventCreate failed");

@@ -90,6 +90,15 @@ fij_status fij_mem_fill_u64(void* ptr, uint64_t value, uint64_t count, void* str
   return FIJ_OK;
 }
 
+fij_status fij_copy_kernel(void* dst, const void* src, uint64_t bytes, void* stream) {
+  if (bytes == 0) return FIJ_OK;
+  if (!dst || !src) return fij::set_error(FIJ_INVALID_ARGUMENT, "NULL buffer");
+  if (bytes % 16 || reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+    return fij::set_error(FIJ_INVALID_ARGUMENT, "copy kernel needs 16-byte aligned buffers and sizes");
+  RT_HIP(fij::launch_copy(dst, src, bytes / 16, static_cast<hipStream_t>(stream)));
+  return FIJ_OK;
+}
+
 fij_status fij_stream_create(void** out) {
   if (!out) return fij::set_error(FIJ_INVALID_ARGUMENT, "out is NULL");
   hipStream_t s = nullptr;

@@ -188,6 +188,7 @@ _proto("fij_memcpy", C.c_int, _vp, _vp, _u64, _vp)
 _proto("fij_memcpy_async", C.c_int, _vp, _vp, _u64, _vp)
 _proto("fij_mem_fill_u32", C.c_int, _vp, _u32, _u64, _vp)
 _proto("fij_mem_fill_u64", C.c_int, _vp, _u64, _u64, _vp)
+_proto("fij_copy_kernel", C.c_int, _vp, _vp, _u64, _vp)
 _proto("fij_stream_create", C.c_int, C.POINTER(_vp))
 _proto("fij_stream_destroy", C.c_int, _vp)
 _proto("fij_stream_synchronize", C.c_int, _vp)
@@ -206,6 +207,12 @@ def device_count():
 
 def synchronize():
     check(lib.fij_device_synchronize())
+
+
+def copy_kernel(dst, src, nbytes=None, stream=None):
+    """The library's plain device copy kernel (bandwidth reference)."""
+    nbytes = src.nbytes() if nbytes is None else nbytes
+    check(lib.fij_copy_kernel(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, _stream(stream)))
 
 
 class DeviceArray:

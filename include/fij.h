@@ -150,6 +150,11 @@ fij_status fij_memcpy(void *dst, const void *src, uint64_t bytes, void *stream);
 fij_status fij_memcpy_async(void *dst, const void *src, uint64_t bytes, void *stream);
 fij_status fij_mem_fill_u32(void *ptr, uint32_t value, uint64_t count, void *stream);
 fij_status fij_mem_fill_u64(void *ptr, uint64_t value, uint64_t count, void *stream);
+/* A plain device copy kernel (16-byte loads and stores; buffers and size
+ * 16-byte aligned), asynchronous on `stream`: the HBM bandwidth reference
+ * measured beside the partition and join kernels (no counterpart in the
+ * reference, whose measurements quote the nominal peak). */
+fij_status fij_copy_kernel(void *dst, const void *src, uint64_t bytes, void *stream);
 fij_status fij_stream_create(void **out);
 fij_status fij_stream_destroy(void *stream);
 fij_status fij_stream_synchronize(void *stream);
