@@ -86,21 +86,22 @@ __global__ void fill_kernel(U* out, uint64_t n, U value) {
     out[i] = value;
 }
 
-// Plain device copy of 16-byte words, four independent loads in flight per
-// thread: the bandwidth reference the partition and join kernels are compared
-// with, and the known-size kernel that calibrates the HBM byte counters.
-__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+// Plain device copy of 16-byte words: each block copies 4 * 256 consecutive
+// words, every thread four independent non-temporal loads, then four
+// stores.  The bandwidth reference the partition and join kernels are
+// compared with, and the known-size kernel that calibrates the HBM byte
+// counters.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void copy_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
                                                    uint64_t n) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], e = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = e;
-  }
-  for (; i < n; i += stride) dst[i] = src[i];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * 1024 + threadIdx.x;
+  u32x4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k * 256 < n) v[k] = __builtin_nontemporal_load(src + base + k * 256);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k * 256 < n) __builtin_nontemporal_store(v[k], dst + base + k * 256);
 }
 
 uint32_t grid_for(uint64_t n) {
@@ -148,8 +149,10 @@ hipError_t launch_fill_u64(uint64_t* out, uint64_t n, uint64_t value, hipStream_
 }
 
 hipError_t launch_copy(void* dst, const void* src, uint64_t words16, hipStream_t stream) {
-  hipLaunchKernelGGL(copy_kernel, dim3(grid_for(words16)), dim3(256), 0, stream, static_cast<const uint4*>(src),
-                     static_cast<uint4*>(dst), words16);
+  const uint64_t blocks = (words16 + 1023) / 1024;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(copy_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream,
+                     static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), words16);
   return hipGetLastError();
 }
 

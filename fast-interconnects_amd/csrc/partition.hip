@@ -122,16 +122,23 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   const K* keys = static_cast<const K*>(p.keys) + begin;
   const K* pays = static_cast<const K*>(p.pays) + begin;
 
-  for (uint64_t t0 = 0; t0 < len; t0 += TILE) {
-    K key[V], pay[V];
+  // Tile t0's keys and payloads; lanes past the chunk's end re-read its last
+  // element, so every tile issues the same unconditional loads and the next
+  // tile's loads stay in flight while this tile is ranked, placed and flushed.
+  auto load_tile = [&](K (&key)[V], K (&pay)[V], uint64_t t0) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const uint64_t e = min(t0 + static_cast<uint64_t>(k) * BLOCK + tid, len - 1);
+      key[k] = __builtin_nontemporal_load(keys + e);
+      pay[k] = __builtin_nontemporal_load(pays + e);
+    }
+  };
+  auto process = [&](const K (&key)[V], const K (&pay)[V], uint64_t t0) {
     uint32_t d[V], r[V];
     bool act[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-      const uint64_t e = t0 + static_cast<uint64_t>(k) * BLOCK + tid;
-      act[k] = e < len;
-      key[k] = act[k] ? __builtin_nontemporal_load(keys + e) : K(0);
-      pay[k] = act[k] ? __builtin_nontemporal_load(pays + e) : K(0);
+      act[k] = t0 + static_cast<uint64_t>(k) * BLOCK + tid < len;
       d[k] = key_to_partition<K>(key[k], mask, p.ignore_bits);
     }
     // A: rank
@@ -202,6 +209,19 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
     }
     if (tid == 0) qcount = 0;
     __syncthreads();
+  };
+
+  if (len) {
+    K ka[V], pa[V], kb[V], pb[V];
+    load_tile(ka, pa, 0);
+    for (uint64_t t0 = 0;; t0 += 2 * TILE) {
+      load_tile(kb, pb, t0 + TILE);
+      process(ka, pa, t0);
+      if (t0 + TILE >= len) break;
+      load_tile(ka, pa, t0 + 2 * TILE);
+      process(kb, pb, t0 + TILE);
+      if (t0 + 2 * TILE >= len) break;
+    }
   }
   // Write the partially filled last line of every partition.
   for (uint32_t idx = tid; idx < F * L; idx += BLOCK) {

@@ -33,35 +33,12 @@ namespace {
 
 // --------------------------------------------------------------- plan ----
 
-// One block per pass-1 partition p: piece c of p is chunk c's range of p.
-// piece_start[p][c] = logical start of piece c within p; [p][G1] = |p|.
-__global__ __launch_bounds__(256) void plan_pieces_kernel(const uint64_t* off1, uint32_t F1, uint32_t G1,
-                                                          uint32_t pad, uint64_t padded_len, uint32_t* piece_start,
-                                                          uint32_t* plen) {
-  extern __shared__ uint32_t lens[];
-  __shared__ uint32_t wave_tmp[32];
-  const uint32_t p = blockIdx.x;
-  const uint64_t olen = static_cast<uint64_t>(F1) * G1;
-  for (uint32_t c = threadIdx.x; c < G1; c += blockDim.x) {
-    const uint64_t ofi = static_cast<uint64_t>(c) * F1 + p;
-    const uint64_t b = off1[ofi];
-    const uint64_t e = (ofi + 1 < olen) ? off1[ofi + 1] - pad : padded_len;
-    lens[c] = static_cast<uint32_t>(e - b);
-  }
-  __syncthreads();
-  const uint32_t total = block_exclusive_scan<uint32_t>(lens, G1, wave_tmp);
-  uint32_t* out = piece_start + static_cast<uint64_t>(p) * (G1 + 1);
-  for (uint32_t c = threadIdx.x; c < G1; c += blockDim.x) out[c] = lens[c];
-  if (threadIdx.x == 0) {
-    out[G1] = total;
-    plen[p] = total;
-  }
-}
-
-// One block: partition starts in the concatenated pass-2 output and the tile
-// map (tile_start[p] = first tile of p); also resets the join's work counter
-// and aggregates.  blockIdx.x selects the relation.
-__global__ __launch_bounds__(1024) void plan_tiles_kernel(PipelineDevice d) {
+// Partition lengths, starts in the concatenated pass-2 output and the tile
+// map (tile_start[p] = first tile of p) of one relation per block
+// (blockIdx.x selects it); block 0 also resets the join's work counters and
+// aggregates.  Pass-1 partition p of the contiguous layout is the range
+// [off1[p], off1[p + 1] - pad), the last one ends at padded_len1.
+__global__ __launch_bounds__(1024) void plan_kernel(PipelineDevice d) {
   extern __shared__ uint64_t buf[];
   __shared__ uint64_t wave_tmp[32];
   const uint32_t rel = blockIdx.x;
@@ -70,7 +47,10 @@ __global__ __launch_bounds__(1024) void plan_tiles_kernel(PipelineDevice d) {
   uint64_t* starts = buf;              // [F1]
   uint64_t* tiles = buf + F1;          // [F1]
   for (uint32_t p = threadIdx.x; p < F1; p += blockDim.x) {
-    const uint32_t n = s.plen[p];
+    const uint64_t b = s.off1[p];
+    const uint64_t e = p + 1 < F1 ? s.off1[p + 1] - d.pad : s.padded_len1;
+    const uint32_t n = static_cast<uint32_t>(e - b);
+    s.plen[p] = n;
     starts[p] = n;
     tiles[p] = d.bits2 ? (n + d.tile2 - 1) / d.tile2 : 0;
   }
@@ -95,101 +75,111 @@ __global__ __launch_bounds__(1024) void plan_tiles_kernel(PipelineDevice d) {
   }
 }
 
+// Wave-uniform read of data that no launch in flight writes, through the
+// scalar data cache: it is counted apart from the vector loads, so waiting
+// for it does not wait for loads already in flight.
+template <typename U>
+__device__ __forceinline__ U uniform_load(const U* ptr) {
+  return *(const __attribute__((address_space(4))) U*)(ptr);
+}
+
 // ------------------------------------------------------------- sort2 ----
 
+// One pass-2 tile: relation, pass-1 partition p, tile j of p, its length and
+// its first tuple in rel1.
+template <typename K>
+struct TileRef {
+  uint32_t rel, p, j, n;
+  const Tuple<K>* src;
+};
+
+// Pass 2 of both relations as one persistent launch: block b sorts tiles b,
+// b + grid, ... of the concatenated tile lists (build tiles first).  A tile
+// is an LDS counting sort of up to TILE2 tuples of one pass-1 partition by
+// the pass-2 digit: one coalesced read, one coalesced write, and per digit
+// its run bounds in meta.  The next tile's tuples are loaded while the
+// current tile is ranked, sorted and written.
 template <typename K, uint32_t TILE2, uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d, uint32_t rel) {
+__global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
   constexpr uint32_t ITEMS = TILE2 / BLOCK;
   using T = Tuple<K>;
   extern __shared__ uint64_t lds_u64[];
-  T* sorted = reinterpret_cast<T*>(lds_u64);                        // [TILE2]
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(sorted + TILE2);      // [F2]
-  uint32_t* pstart_l = cnt + d.F2;                                   // [G1 + 1] local piece starts
-  uint64_t* psrc_l = reinterpret_cast<uint64_t*>(pstart_l + d.G1 + 2);  // [G1 + 1] (8-aligned: G1 even)
-  __shared__ uint32_t s_p, s_c0, s_np;
+  T* sorted = reinterpret_cast<T*>(lds_u64);                    // [TILE2]
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(sorted + TILE2);  // [F2]
   __shared__ uint32_t wave_tmp[32];
-  const PipelineSide& s = rel == 0 ? d.build : d.probe;
-  const uint32_t F1 = d.F1, F2 = d.F2, G1 = d.G1, tid = threadIdx.x;
-  const uint32_t b = blockIdx.x;
-  if (b >= s.tile_start[F1]) return;  // uniform: grid is an upper bound
-  if (tid == 0) {
-    uint32_t lo = 0, hi = F1;  // last p with tile_start[p] <= b
+  const uint32_t F1 = d.F1, F2 = d.F2, tid = threadIdx.x;
+  const uint32_t tiles_b = uniform_load(d.build.tile_start + F1);
+  const uint32_t total = tiles_b + uniform_load(d.probe.tile_start + F1);
+  const uint64_t mask2 = radix_mask(d.bits2, d.bits1);
+  uint32_t t = blockIdx.x;
+  if (t >= total) return;  // uniform
+  for (uint32_t i = tid; i < F2; i += BLOCK) cnt[i] = 0;
+  auto locate = [&](uint32_t tile) {
+    TileRef<K> r;
+    r.rel = tile >= tiles_b;
+    const PipelineSide& s = r.rel ? d.probe : d.build;
+    const uint32_t u = r.rel ? tile - tiles_b : tile;
+    uint32_t lo = 0, hi = F1;  // last p with tile_start[p] <= u (skips empty partitions)
     while (hi - lo > 1) {
       const uint32_t m = (lo + hi) / 2;
-      if (s.tile_start[m] <= b) lo = m; else hi = m;
+      if (uniform_load(s.tile_start + m) <= u) lo = m; else hi = m;
     }
-    s_p = lo;
-  }
-  for (uint32_t i = tid; i < F2; i += BLOCK) cnt[i] = 0;
-  __syncthreads();
-  const uint32_t p = s_p;
-  const uint32_t j = b - s.tile_start[p];
-  const uint32_t n_p = s.plen[p];
-  const uint32_t lo = j * TILE2;
-  const uint32_t n = min(TILE2, n_p - lo);
-  const uint32_t* ps = s.piece_start + static_cast<uint64_t>(p) * (G1 + 1);
-  if (tid == 0) {
-    // pieces overlapping [lo, lo + n): first c with ps[c+1] > lo
-    uint32_t l = 0, h = G1;
-    while (h - l > 1) {
-      const uint32_t m = (l + h) / 2;
-      if (ps[m] <= lo) l = m; else h = m;
-    }
-    uint32_t c1 = l;
-    while (c1 < G1 && ps[c1] < lo + n) ++c1;
-    s_c0 = l;
-    s_np = c1 - l;
-  }
-  __syncthreads();
-  const uint32_t c0 = s_c0, np = s_np;
-  for (uint32_t i = tid; i <= np; i += BLOCK) {
-    pstart_l[i] = ps[c0 + i];
-    if (i < np) psrc_l[i] = s.off1[static_cast<uint64_t>(c0 + i) * F1 + p];
-  }
-  __syncthreads();
-  const T* src = static_cast<const T*>(s.rel1);
-  const uint64_t mask2 = radix_mask(d.bits2, d.bits1);
-  K key[ITEMS], val[ITEMS];
-  uint32_t dig[ITEMS], rank[ITEMS];
+    r.p = lo;
+    r.j = u - uniform_load(s.tile_start + lo);
+    r.n = min(TILE2, uniform_load(s.plen + lo) - r.j * TILE2);
+    r.src = static_cast<const T*>(s.rel1) + uniform_load(s.off1 + lo) + static_cast<uint64_t>(r.j) * TILE2;
+    return r;
+  };
+  // lanes past the tile's end re-read its last tuple: unconditional loads
+  auto load = [&](const TileRef<K>& r, T (&v)[ITEMS]) {
 #pragma unroll
-  for (uint32_t k = 0; k < ITEMS; ++k) {
-    const uint32_t x = lo + k * BLOCK + tid;
-    const bool act = x < lo + n;
-    key[k] = 0;
-    val[k] = 0;
-    if (act) {
-      uint32_t l = 0, h = np;  // piece i with pstart_l[i] <= x < pstart_l[i+1]
-      while (h - l > 1) {
-        const uint32_t m = (l + h) / 2;
-        if (pstart_l[m] <= x) l = m; else h = m;
-      }
-      const T t = load_tuple_nt<K>(src + psrc_l[l] + (x - pstart_l[l]));
-      key[k] = t.key;
-      val[k] = t.value;
-    }
-    dig[k] = key_to_partition<K>(key[k], mask2, d.bits1);
-    rank[k] = wave_rank_add(cnt, dig[k], act);
-  }
-  __syncthreads();
-  block_exclusive_scan<uint32_t>(cnt, F2, wave_tmp);
-  // per-tile run bounds of every pass-2 digit: start | end << 16 (TILE2 <= 2^16)
-  uint32_t* meta = s.meta + static_cast<uint64_t>(s.tile_start[p] + j) * F2;
-  for (uint32_t i = tid; i < F2; i += BLOCK) meta[i] = cnt[i] | ((i + 1 < F2 ? cnt[i + 1] : n) << 16);
+    for (uint32_t k = 0; k < ITEMS; ++k) v[k] = load_tuple_nt<K>(r.src + min(k * BLOCK + tid, r.n - 1));
+  };
+  auto process = [&](const TileRef<K>& r, const T (&v)[ITEMS]) {
+    const PipelineSide& s = r.rel ? d.probe : d.build;
+    uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
-  for (uint32_t k = 0; k < ITEMS; ++k) {
-    const uint32_t x = lo + k * BLOCK + tid;
-    if (x < lo + n) {
-      T t;
-      t.key = key[k];
-      t.value = val[k];
-      sorted[cnt[dig[k]] + rank[k]] = t;
+    for (uint32_t k = 0; k < ITEMS; ++k) {
+      const bool act = k * BLOCK + tid < r.n;
+      dig[k] = key_to_partition<K>(v[k].key, mask2, d.bits1);
+      rank[k] = wave_rank_add(cnt, dig[k], act);
     }
-  }
+    __syncthreads();
+    block_exclusive_scan<uint32_t>(cnt, F2, wave_tmp);
+    // run bounds of every pass-2 digit in this tile: start | end << 16 (TILE2 <= 2^16)
+    uint32_t* meta = s.meta + static_cast<uint64_t>(uniform_load(s.tile_start + r.p) + r.j) * F2;
+    for (uint32_t i = tid; i < F2; i += BLOCK) meta[i] = cnt[i] | ((i + 1 < F2 ? cnt[i + 1] : r.n) << 16);
+#pragma unroll
+    for (uint32_t k = 0; k < ITEMS; ++k)
+      if (k * BLOCK + tid < r.n) sorted[cnt[dig[k]] + rank[k]] = v[k];
+    __syncthreads();
+    T* out = static_cast<T*>(s.rel2) + uniform_load(s.pstart + r.p) + static_cast<uint64_t>(r.j) * TILE2;
+    for (uint32_t i = tid; i < r.n; i += BLOCK) {
+      const T x = sorted[i];
+      store_tuple_nt<K>(out + i, x.key, x.value);
+    }
+    for (uint32_t i = tid; i < F2; i += BLOCK) cnt[i] = 0;
+    __syncthreads();
+  };
+  TileRef<K> cur = locate(t);
+  T va[ITEMS], vb[ITEMS];
+  load(cur, va);
   __syncthreads();
-  T* out = static_cast<T*>(s.rel2) + s.pstart[p] + lo;
-  for (uint32_t i = tid; i < n; i += BLOCK) {
-    const T t = sorted[i];
-    store_tuple<K>(out + i, t.key, t.value);
+  // one step: prefetch the next tile into `nbuf`, sort the current one from
+  // `cbuf`; false when this block has no tile left
+  auto step = [&](T (&cbuf)[ITEMS], T (&nbuf)[ITEMS]) -> bool {
+    const uint32_t tn = t + gridDim.x;
+    const bool more = tn < total;
+    const TileRef<K> nx = more ? locate(tn) : cur;
+    load(nx, nbuf);
+    process(cur, cbuf);
+    t = tn;
+    cur = nx;
+    return more;
+  };
+  for (;;) {
+    if (!step(va, vb)) break;
+    if (!step(vb, va)) break;
   }
 }
 
@@ -197,23 +187,15 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d, uint32_t
 
 constexpr uint32_t kMaxRuns = 64;
 
-// Wave-uniform read of data that no launch in flight writes, through the
-// scalar data cache: it is counted apart from the vector loads, so waiting
-// for it does not wait for gathers already in flight.
-template <typename U>
-__device__ __forceinline__ U uniform_load(const U* ptr) {
-  return *(const __attribute__((address_space(4))) U*)(ptr);
-}
-
 __device__ __forceinline__ uint32_t run_count(const PipelineDevice& d, const PipelineSide& s, uint32_t p) {
-  return d.bits2 ? uniform_load(s.tile_start + p + 1) - uniform_load(s.tile_start + p) : d.G1;
+  return d.bits2 ? uniform_load(s.tile_start + p + 1) - uniform_load(s.tile_start + p) : 1u;
 }
 
 // Loaded, not yet combined, descriptor of run r0 + lane of item (p, q).  With
 // two passes run r is pass-2 digit q's range inside tile r of pass-1
 // partition p: a = first tuple of the tile, b0 = its meta word (start | end
-// << 16).  With one pass run r is chunk r's piece of p: a = its first tuple,
-// [b0, b1) = its logical bounds.  Every lane loads from an in-bounds address
+// << 16).  With one pass the item is partition p, one run: a = its first
+// tuple, [b0, b1) = [0, its length).  Every lane loads from an in-bounds address
 // (lanes past the last run repeat it) and combining is left to the consumer:
 // no branch and no use follows the loads, so a prefetch does not make the
 // gathers issued before it wait.
@@ -234,10 +216,9 @@ __device__ __forceinline__ RawRun raw_run(const PipelineDevice& d, const Pipelin
     x.b0 = s.meta[static_cast<uint64_t>(tile) * d.F2 + q];
     x.b1 = 0;
   } else {
-    const uint32_t* ps = s.piece_start + static_cast<uint64_t>(p) * (d.G1 + 1);
-    x.a = s.off1[static_cast<uint64_t>(r) * d.F1 + p];
-    x.b0 = ps[r];
-    x.b1 = ps[r + 1];
+    x.a = s.off1[p];
+    x.b0 = 0;
+    x.b1 = s.plen[p];
   }
   return x;
 }
@@ -686,9 +667,9 @@ size_t join_lds_bytes() {
   return WAVES * join_wave_bytes<K, CAP>();
 }
 
-template <typename K, uint32_t TILE2, uint32_t BLOCK>
-size_t sort2_lds_bytes(uint32_t F2, uint32_t G1) {
-  return TILE2 * sizeof(Tuple<K>) + F2 * 4 + (G1 + 2) * 4 + (G1 + 2) * 8 + 16;
+template <typename K, uint32_t TILE2>
+size_t sort2_lds_bytes(uint32_t F2) {
+  return TILE2 * sizeof(Tuple<K>) + F2 * 4 + 16;
 }
 
 }  // namespace
@@ -698,39 +679,31 @@ size_t sort2_lds_bytes(uint32_t F2, uint32_t G1) {
 uint32_t pipeline_tile2(KeyType t) { (void)t; return 8192u; }
 
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream) {
-  for (int rel = 0; rel < 2; ++rel) {
-    const PipelineSide& s = rel == 0 ? d.build : d.probe;
-    hipLaunchKernelGGL(plan_pieces_kernel, dim3(d.F1), dim3(256), d.G1 * 4 + 16, stream, s.off1, d.F1, d.G1,
-                       d.pad, s.padded_len1, s.piece_start, s.plen);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(plan_tiles_kernel, dim3(2), dim3(1024), 2 * d.F1 * 8, stream, d);
+  hipLaunchKernelGGL(plan_kernel, dim3(2), dim3(1024), 2 * d.F1 * 8, stream, d);
   return hipGetLastError();
 }
 
 template <typename K>
-hipError_t launch_sort2_t(const PipelineDevice& d, uint32_t rel, uint32_t max_tiles, hipStream_t stream) {
+hipError_t launch_sort2_t(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
   constexpr uint32_t TILE2 = 8192;
   constexpr uint32_t BLOCK = 1024;
   auto kern = sort2_kernel<K, TILE2, BLOCK>;
-  const size_t lds = sort2_lds_bytes<K, TILE2, BLOCK>(d.F2, d.G1);
+  const size_t lds = sort2_lds_bytes<K, TILE2>(d.F2);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));
-  hipLaunchKernelGGL(kern, dim3(max_tiles), dim3(BLOCK), lds, stream, d, rel);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, stream, d);
   return hipGetLastError();
 }
 
-hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t rel, uint32_t max_tiles,
-                                 hipStream_t stream) {
-  if (max_tiles == 0) return hipSuccess;
-  if (t == KeyType::I32) return launch_sort2_t<int32_t>(d, rel, max_tiles, stream);
-  return launch_sort2_t<int64_t>(d, rel, max_tiles, stream);
+hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t grid, hipStream_t stream) {
+  if (grid == 0) return hipSuccess;
+  if (t == KeyType::I32) return launch_sort2_t<int32_t>(d, grid, stream);
+  return launch_sort2_t<int64_t>(d, grid, stream);
 }
 
-size_t pipeline_sort2_lds(KeyType t, uint32_t F2, uint32_t G1) {
-  if (t == KeyType::I32) return sort2_lds_bytes<int32_t, 8192, 1024>(F2, G1);
-  return sort2_lds_bytes<int64_t, 8192, 1024>(F2, G1);
+size_t pipeline_sort2_lds(KeyType t, uint32_t F2) {
+  if (t == KeyType::I32) return sort2_lds_bytes<int32_t, 8192>(F2);
+  return sort2_lds_bytes<int64_t, 8192>(F2);
 }
 
 template <typename K, uint32_t CAP, uint32_t WAVES>
@@ -748,12 +721,6 @@ hipError_t launch_join_t(const PipelineDevice& d, uint32_t grid, hipStream_t str
                             static_cast<int>(lds));
   hipLaunchKernelGGL(gen, dim3(grid), dim3(WAVES * 64), lds, stream, d);
   return hipGetLastError();
-}
-
-uint32_t pipeline_join_blocks_per_cu(KeyType t, bool large) {
-  (void)t;
-  (void)large;
-  return 2;
 }
 
 hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, uint32_t grid,

@@ -9,18 +9,22 @@
 namespace fij {
 
 // One relation's buffers.  rel1 is the pass-1 partitioned relation in the
-// chunked layout (G1 chunks x F1 partitions, 128-byte padding in front of
-// every partition) with offsets off1[G1][F1]; rel2 holds the pass-2 output:
-// the concatenation of all pass-1 partitions, each cut into tiles of tile2
-// tuples that are sorted by the pass-2 digit.
+// contiguous layout (each partition one range, 128 bytes of padding in front
+// of it): chunk c writes partition p from local1[c][p] on, partition p starts
+// at off1[p].  rel2 holds the pass-2 output: the concatenation of all pass-1
+// partitions, each cut into tiles of tile2 tuples that are sorted by the
+// pass-2 digit.
 struct PipelineSide {
   const void* keys;           // input columns
   const void* pays;
   uint64_t n;
-  uint64_t* off1;             // [G1 * F1]
-  void* rel1;                 // Tuple[n + F1 * G1 * pad]
-  uint64_t padded_len1;
-  uint32_t* piece_start;      // [F1][G1 + 1]
+  uint64_t* off1;             // [F1] partition starts in rel1
+  uint64_t* local1;           // [G1][F1] per-chunk write offsets
+  uint32_t* counts1;          // [F1][G1] per-chunk histograms
+  uint64_t* scan1;            // look-back scan state of the offsets scan
+  uint64_t scan1_len;
+  void* rel1;                 // Tuple[padded_len1]
+  uint64_t padded_len1;       // n + F1 * pad
   uint32_t* plen;             // [F1]
   uint64_t* pstart;           // [F1 + 1] start of partition p in rel2
   uint32_t* tile_start;       // [F1 + 1] first pass-2 tile of partition p
@@ -50,10 +54,10 @@ struct PipelineDevice {
 };
 
 uint32_t pipeline_tile2(KeyType t);
-size_t pipeline_sort2_lds(KeyType t, uint32_t F2, uint32_t G1);
+size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
-hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t rel, uint32_t max_tiles,
-                                 hipStream_t stream);
+// Pass 2 of both relations in one persistent launch of `grid` blocks.
+hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t grid, hipStream_t stream);
 hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, uint32_t grid,
                                 hipStream_t stream);
 

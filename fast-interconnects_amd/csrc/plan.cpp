@@ -27,6 +27,7 @@ struct fij_join_plan {
   KeyType kt;
   PipelineDevice d;
   uint32_t max_tiles[2];
+  uint32_t sort2_grid;
   uint32_t join_grid;
   uint32_t block1;
   uint32_t lds_total;
@@ -80,13 +81,16 @@ fij_status alloc_side(fij_join_plan* p, PipelineSide& s, uint64_t n, uint32_t ti
   const uint32_t F1 = p->d.F1, G1 = p->d.G1, F2 = p->d.F2;
   const uint32_t tb = tuple_bytes(p->kt);
   s.n = n;
-  s.padded_len1 = n + static_cast<uint64_t>(F1) * G1 * p->d.pad;
+  s.padded_len1 = n + static_cast<uint64_t>(F1) * p->d.pad;
+  s.scan1_len = scan_state_words(static_cast<uint64_t>(F1) * G1);
   fij_status st;
-  if ((st = alloc(p, &s.off1, static_cast<uint64_t>(F1) * G1))) return st;
+  if ((st = alloc(p, &s.off1, F1))) return st;
+  if ((st = alloc(p, &s.local1, static_cast<uint64_t>(G1) * F1))) return st;
+  if ((st = alloc(p, &s.counts1, static_cast<uint64_t>(F1) * G1))) return st;
+  if ((st = alloc(p, &s.scan1, s.scan1_len))) return st;
   uint8_t* rel1 = nullptr;
   if ((st = alloc(p, &rel1, s.padded_len1 * tb))) return st;
   s.rel1 = rel1;
-  if ((st = alloc(p, &s.piece_start, static_cast<uint64_t>(F1) * (G1 + 1)))) return st;
   if ((st = alloc(p, &s.plen, F1))) return st;
   if ((st = alloc(p, &s.pstart, F1 + 1))) return st;
   if ((st = alloc(p, &s.tile_start, F1 + 1))) return st;
@@ -137,7 +141,6 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
   p->d.F1 = 1u << cfg->radix_bits_1;
   p->d.F2 = 1u << cfg->radix_bits_2;
   p->d.G1 = cfg->pass1_chunks ? cfg->pass1_chunks : cus;
-  p->d.G1 += p->d.G1 & 1;  // even, keeps the sort's LDS piece table 8-byte aligned
   p->d.pad = pad_tuples(kt);
   p->d.tile2 = pipeline_tile2(kt);
   p->block1 = cfg->pass1_block ? std::min<uint32_t>(cfg->pass1_block, 1024) : 1024;
@@ -152,13 +155,17 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
     if (st == FIJ_OK && chunk * (p->d.G1 - 1) > lens[rel])
       st = plan_fail(FIJ_INVALID_ARGUMENT, "relation of %llu tuples is too short for %u chunks",
                      static_cast<unsigned long long>(lens[rel]), p->d.G1);
-    const uint64_t last = lens[rel] - chunk * (p->d.G1 - 1);
-    if (st == FIJ_OK && std::max(chunk, last) + static_cast<uint64_t>(p->d.F1) * p->d.pad * 2 >= 0xFFFFFFFFull)
-      st = plan_fail(FIJ_INTEGER_OVERFLOW, "pass-1 chunks exceed 32-bit offsets; use more chunks");
+    // the scatter addresses a chunk's writes relative to its first partition
+    // with 32-bit offsets; in the contiguous layout they span the relation
+    if (st == FIJ_OK && lens[rel] + static_cast<uint64_t>(p->d.F1) * p->d.pad * 2 >= 0xFFFFFFFFull)
+      st = plan_fail(FIJ_INTEGER_OVERFLOW, "relation of %llu tuples exceeds the 32-bit pass-1 offsets",
+                     static_cast<unsigned long long>(lens[rel]));
     p->max_tiles[rel] = p->d.bits2 ? static_cast<uint32_t>((lens[rel] + p->d.tile2 - 1) / p->d.tile2 + p->d.F1) : 0;
   }
-  if (st == FIJ_OK && p->d.bits2 && pipeline_sort2_lds(kt, p->d.F2, p->d.G1) > lds_total)
+  if (st == FIJ_OK && p->d.bits2 && pipeline_sort2_lds(kt, p->d.F2) > lds_total)
     st = plan_fail(FIJ_UNSUPPORTED, "pass-2 tile does not fit LDS for this fanout / chunk count");
+  // pass 2: one persistent block per compute unit (its LDS tile fills the CU)
+  p->sort2_grid = p->d.bits2 ? std::min<uint32_t>(cus, p->max_tiles[0] + p->max_tiles[1]) : 0;
   if (st == FIJ_OK) st = alloc_side(p, p->d.build, cfg->build_len, p->max_tiles[0]);
   if (st == FIJ_OK) st = alloc_side(p, p->d.probe, cfg->probe_len, p->max_tiles[1]);
   if (st == FIJ_OK) st = alloc(p, &p->d.work_counter, kJoinCounterWords);
@@ -206,8 +213,11 @@ fij_status fij_join_plan_enqueue(fij_join_plan* p, const void* build_keys, const
   d.probe.keys = probe_keys;
   d.probe.pays = probe_pays;
   PLAN_HIP(hipEventRecord(p->ev[0], st));
-  for (PipelineSide* s : {&d.build, &d.probe})
-    PLAN_HIP(launch_hist_chunked(p->kt, s->keys, s->n, d.G1, d.bits1, 0, s->off1, p->block1, st));
+  for (PipelineSide* s : {&d.build, &d.probe}) {
+    PLAN_HIP(launch_hist_counts(p->kt, s->keys, nullptr, nullptr, nullptr, s->n, d.G1, d.bits1, 0, s->counts1,
+                                s->scan1, s->scan1_len, p->block1, st));
+    PLAN_HIP(launch_offsets_scan(s->counts1, d.F1, d.G1, d.pad, s->local1, s->off1, s->scan1, st));
+  }
   PLAN_HIP(hipEventRecord(p->ev[1], st));
   for (PipelineSide* s : {&d.build, &d.probe}) {
     PartitionLaunch a{};
@@ -218,7 +228,7 @@ fij_status fij_join_plan_enqueue(fij_join_plan* p, const void* build_keys, const
     a.chunks = d.G1;
     a.radix_bits = d.bits1;
     a.ignore_bits = 0;
-    a.chunk_offsets = s->off1;
+    a.chunk_offsets = s->local1;
     a.relation = s->rel1;
     a.block = p->block1;
     a.lds_bytes = p->lds_total;
@@ -227,10 +237,7 @@ fij_status fij_join_plan_enqueue(fij_join_plan* p, const void* build_keys, const
   }
   PLAN_HIP(hipEventRecord(p->ev[2], st));
   PLAN_HIP(launch_pipeline_plan(d, st));
-  if (d.bits2) {
-    PLAN_HIP(launch_pipeline_sort2(d, p->kt, 0, p->max_tiles[0], st));
-    PLAN_HIP(launch_pipeline_sort2(d, p->kt, 1, p->max_tiles[1], st));
-  }
+  if (d.bits2) PLAN_HIP(launch_pipeline_sort2(d, p->kt, p->sort2_grid, st));
   PLAN_HIP(hipEventRecord(p->ev[3], st));
   PLAN_HIP(launch_pipeline_join(d, p->kt, p->large_tables, p->join_grid, st));
   PLAN_HIP(hipEventRecord(p->ev[4], st));
@@ -284,8 +291,8 @@ fij_status fij_join_plan_device_bytes(const fij_join_plan* p, uint64_t* bytes) {
   const uint64_t tb = tuple_bytes(p->kt);
   uint64_t total = 0;
   for (const PipelineSide* s : {&p->d.build, &p->d.probe}) {
-    total += s->padded_len1 * tb + static_cast<uint64_t>(p->d.F1) * p->d.G1 * 8;
-    total += static_cast<uint64_t>(p->d.F1) * (p->d.G1 + 1) * 4 + p->d.F1 * 4 + (p->d.F1 + 1) * 12;
+    total += s->padded_len1 * tb + static_cast<uint64_t>(p->d.F1) * (p->d.G1 + 1) * 8;
+    total += static_cast<uint64_t>(p->d.F1) * p->d.G1 * 4 + s->scan1_len * 8 + p->d.F1 * 4 + (p->d.F1 + 1) * 12;
     if (p->d.bits2) total += s->n * tb;
   }
   if (p->d.bits2) total += (static_cast<uint64_t>(p->max_tiles[0]) + p->max_tiles[1] + 2) * p->d.F2 * 4;
