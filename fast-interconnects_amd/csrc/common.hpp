@@ -152,6 +152,18 @@ __device__ __forceinline__ uint64_t wave_inclusive_scan(uint64_t x) {
   return x;
 }
 
+// Inclusive max-scan over the 64 lanes (same DPP pattern as the sum scan;
+// lanes a DPP step does not reach combine with 0).
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t x) {
+  x = max(x, dpp_u32<0x111, 0xf>(x));
+  x = max(x, dpp_u32<0x112, 0xf>(x));
+  x = max(x, dpp_u32<0x114, 0xf>(x));
+  x = max(x, dpp_u32<0x118, 0xf>(x));
+  x = max(x, dpp_u32<0x142, 0xa>(x));
+  x = max(x, dpp_u32<0x143, 0xc>(x));
+  return x;
+}
+
 template <typename U>
 __device__ __forceinline__ U wave_broadcast_last(U x) {
   if constexpr (sizeof(U) == 4) {

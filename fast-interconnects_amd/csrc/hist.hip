@@ -95,22 +95,45 @@ __global__ __launch_bounds__(kHistBlockMax) void hist_kernel(HistParams p) {
   if (aligned) {
     const uint64_t nv = len / V::N;
     const typename V::type* kv = reinterpret_cast<const typename V::type*>(keys);
-    for (uint64_t v0 = 0; v0 < nv; v0 += B) {
-      const uint64_t v = v0 + tid;
-      const bool act = v < nv;
-      typename V::type x = {};
-      if (act) x = __builtin_nontemporal_load(kv + v);
-      if constexpr (COPY) {
-        if (act) {
-          const typename V::type* pv =
-              reinterpret_cast<const typename V::type*>(static_cast<const K*>(p.pays) + begin);
-          typename V::type y = __builtin_nontemporal_load(pv + v);
-          reinterpret_cast<typename V::type*>(static_cast<K*>(p.dst_keys) + begin)[v] = x;
-          reinterpret_cast<typename V::type*>(static_cast<K*>(p.dst_pays) + begin)[v] = y;
-        }
-      }
+    // U vectors per thread and step; the next step's vectors are loaded while
+    // this step's keys are counted.  Lanes past the end re-read the last
+    // vector (unconditional loads) and count nothing.
+    constexpr uint32_t U = 4;
+    using VT = typename V::type;
+    auto load = [&](VT (&x)[U], uint64_t v0) {
 #pragma unroll
-      for (uint32_t j = 0; j < V::N; ++j) wave_count_add(my, key_to_partition<K>(x[j], mask, p.ignore_bits), act);
+      for (uint32_t u = 0; u < U; ++u) x[u] = __builtin_nontemporal_load(kv + min(v0 + u * B + tid, nv - 1));
+    };
+    auto count = [&](const VT (&x)[U], uint64_t v0) {
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint64_t v = v0 + u * B + tid;
+        const bool act = v < nv;
+        if constexpr (COPY) {
+          if (act) {
+            const VT* pv = reinterpret_cast<const VT*>(static_cast<const K*>(p.pays) + begin);
+            const VT y = __builtin_nontemporal_load(pv + v);
+            reinterpret_cast<VT*>(static_cast<K*>(p.dst_keys) + begin)[v] = x[u];
+            reinterpret_cast<VT*>(static_cast<K*>(p.dst_pays) + begin)[v] = y;
+          }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < V::N; ++j)
+          wave_count_add(my, key_to_partition<K>(x[u][j], mask, p.ignore_bits), act);
+      }
+    };
+    if (nv) {
+      VT xa[U], xb[U];
+      const uint64_t step = static_cast<uint64_t>(U) * B;
+      load(xa, 0);
+      for (uint64_t v0 = 0;; v0 += 2 * step) {
+        load(xb, v0 + step);
+        count(xa, v0);
+        if (v0 + step >= nv) break;
+        load(xa, v0 + 2 * step);
+        count(xb, v0 + step);
+        if (v0 + 2 * step >= nv) break;
+      }
     }
     scalar_from = nv * V::N;
   }

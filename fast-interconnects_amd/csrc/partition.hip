@@ -9,18 +9,20 @@
 //
 // The buffered variant is designed for wave64 / 160 KB LDS: instead of the
 // reference's per-buffer spin locks it processes the chunk in block-wide
-// tiles with four barrier-separated phases, so no wave ever waits on another
-// wave's progress inside a phase:
+// tiles with three barrier-separated phases, so no wave ever waits on
+// another wave's progress inside a phase:
 //   A  rank:   wave-aggregated LDS atomics give each tuple its position in
 //              its partition's output sequence (ballot + mbcnt);
 //   B  place:  the tuple goes into its partition's LDS ring of R aligned
-//              128-byte lines; a tuple that completes a line enqueues it.
-//              A tuple beyond the ring's capacity is stored directly;
-//   C  flush:  waves write queued lines, 64/L lines per store instruction,
-//              each a full aligned 128-byte line;
-//   D  commit: per-partition fill / flushed / low-water marks advance.
+//              128-byte lines; a tuple that completes a line enqueues it
+//              (with the partition's low-water mark at that time).  A tuple
+//              beyond the ring's capacity is stored directly;
+//   C  flush + commit: waves write queued lines, 64/L lines per store
+//              instruction, each a full aligned 128-byte line, while the
+//              per-partition fill / flushed / low-water marks advance.
 // Positions below a partition's low-water mark (front alignment slots and
-// slots already stored directly) are never written from the ring.
+// slots already stored directly) are never written from the ring.  The next
+// tile's loads are in flight while a tile goes through the phases.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -99,6 +101,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   uint32_t* cnt = ring_lo + F;    // per-tile arrivals
   uint32_t* qpart = cnt + F;      // [F * R] queued line: partition
   uint32_t* qline = qpart + static_cast<size_t>(F) * R;  // [F * R] queued line: line number
+  uint32_t* qlo = qline + static_cast<size_t>(F) * R;    // [F * R] queued line: partition's low-water mark
 
   const uint64_t* off = p.chunk_offsets + static_cast<uint64_t>(blockIdx.x) * F;
   // The first partition starts at least one padding (= one line) into the
@@ -144,6 +147,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
     // A: rank
 #pragma unroll
     for (int k = 0; k < V; ++k) r[k] = wave_rank_add(cnt, d[k], act[k]);
+    if (tid == 0) qcount = 0;
     __syncthreads();
     // B: place
 #pragma unroll
@@ -174,11 +178,13 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
           const uint32_t q = qb + mbcnt(cm);
           qpart[q] = d[k];
           qline[q] = pos / L;
+          qlo[q] = ring_lo[d[k]];
         }
       }
     }
     __syncthreads();
-    // C: flush full lines
+    // C: flush full lines (reads ring, the queue and base only) and commit
+    // (writes fill, flushed, ring_lo and cnt only)
     {
       const uint32_t nq = qcount;
       const uint32_t w = tid / kWave;
@@ -188,15 +194,13 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
         if (q < nq) {
           const uint32_t pp = qpart[q];
           const uint32_t pos = qline[q] * L + (lane % L);
-          if (pos >= ring_lo[pp]) {
+          if (pos >= qlo[q]) {
             const T t = ring[static_cast<size_t>(pp) * RL + (pos & (RL - 1))];
             store_tuple<K>(rel + base[pp] + pos, t.key, t.value);
           }
         }
       }
     }
-    __syncthreads();
-    // D: commit
     for (uint32_t i = tid; i < F; i += BLOCK) {
       const uint32_t c = cnt[i];
       if (c) {
@@ -207,7 +211,6 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
         cnt[i] = 0;
       }
     }
-    if (tid == 0) qcount = 0;
     __syncthreads();
   };
 
@@ -293,7 +296,7 @@ uint32_t swwc_lds_bytes(uint32_t fanout, uint32_t tuple_bytes, uint32_t ring_lin
   (void)tuple_bytes;
   const uint64_t bytes = static_cast<uint64_t>(fanout) * ring_lines * kAlignBytes  // rings
                          + static_cast<uint64_t>(fanout) * 4 * 5                  // per-partition words
-                         + static_cast<uint64_t>(fanout) * ring_lines * 8;        // line queue
+                         + static_cast<uint64_t>(fanout) * ring_lines * 12;       // line queue
   return bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(bytes);
 }
 

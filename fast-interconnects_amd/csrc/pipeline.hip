@@ -280,6 +280,31 @@ __device__ __forceinline__ uint64_t batch_element(const WaveRuns& wr, uint32_t e
   return wr.start[l] + (e - wr.prefix[l]);
 }
 
+// Gather indices of elements k * 64 + lane (k < N) of a published batch of
+// at most N * 64 tuples, without a search: every non-empty run writes its
+// index at its first element's position of the wave's `marks` (all zero
+// between calls), and the run of element e is the maximum mark at or before
+// e -- a wave max-scan per 64 elements, carried across rounds.  Elements
+// past the batch end repeat its last element.
+template <uint32_t N>
+__device__ __forceinline__ void run_indices(const WaveRuns& wr, uint8_t* marks, const LaneRun& x, uint32_t total,
+                                            uint64_t (&idx)[N]) {
+  const uint32_t lane = lane_id();
+  const uint32_t first = wave_inclusive_scan(x.len) - x.len;
+  if (x.len) marks[first] = static_cast<uint8_t>(lane);
+  wave_lds_sync();
+  uint32_t carry = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < N; ++k) {
+    const uint32_t e = min(k * 64 + lane, total - 1);
+    const uint32_t r = max(carry, wave_inclusive_max(marks[e]));
+    carry = __builtin_amdgcn_readlane(r, 63);
+    idx[k] = wr.start[r] + (e - wr.prefix[r]);
+  }
+  wave_lds_sync();
+  if (x.len) marks[first] = 0;
+}
+
 // Wave-private LDS hash table: linear probing from the multiply-shift hash
 // over CAP (power of two) slots.  KT is the stored key type; a free slot holds
 // all ones.  Values are stored beside the keys.
@@ -332,9 +357,11 @@ struct WaveTable {
   }
 };
 
+constexpr uint32_t kMarkBytes = 640;  // run marks per side: the register-resident batch (10 * 64 tuples)
+
 template <typename K, uint32_t CAP>
 constexpr size_t join_wave_bytes() {
-  return (CAP * 2 * sizeof(K) + 2 * kMaxRuns * 8 + 2 * (kMaxRuns + 1) * 4 + 15) & ~size_t(15);
+  return (CAP * 2 * sizeof(K) + 2 * kMaxRuns * 8 + 2 * (kMaxRuns + 1) * 4 + 2 * kMarkBytes + 15) & ~size_t(15);
 }
 
 // Any item, any run count and size: run batches of 64 per side, table fills
@@ -475,6 +502,7 @@ template <typename K, uint32_t CAP>
 struct WaveLds {
   K* table_keys;
   WaveRuns rb, rp;
+  uint8_t *mb, *mp;  // run marks of the two sides
   __device__ __forceinline__ explicit WaveLds(uint64_t* lds) {
     uint8_t* mine = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x / 64) * join_wave_bytes<K, CAP>();
     table_keys = reinterpret_cast<K*>(mine);
@@ -482,17 +510,22 @@ struct WaveLds {
     rb = WaveRuns{reinterpret_cast<uint64_t*>(runs), reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8)};
     rp = WaveRuns{reinterpret_cast<uint64_t*>(runs + kMaxRuns * 8),
                   reinterpret_cast<uint32_t*>(runs + 2 * kMaxRuns * 8 + (kMaxRuns + 1) * 4)};
+    mb = runs + 2 * kMaxRuns * 8 + 2 * (kMaxRuns + 1) * 4;
+    mp = mb + kMarkBytes;
   }
 };
 
 // The items the register-resident join deferred (more than 64 runs or more
-// tuples than its registers hold on a side): it sets overflow[item] = 1.
-// Waves take tickets of 64 items, join every flagged one (one wave per item)
-// and clear its flag, so the flags are all zero again for the next join.
+// tuples than its registers hold on a side): it sets overflow[item] = 1 and
+// the counter word kOverflowAny.  When that word is zero every block only
+// adds a zero aggregate; otherwise waves take tickets of 256 items, join
+// every flagged one (one wave per item) and clear its flag, so the flags are
+// all zero again for the next join.
 template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL>
 __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice d) {
   using T = Tuple<K>;
   constexpr uint32_t RCAP = CAP * 3 / 4;
+  constexpr uint32_t PER_LANE = 4;  // flags per lane and ticket
   extern __shared__ uint64_t lds_u64[];
   const WaveLds<K, CAP> l(lds_u64);
   const uint32_t lane = lane_id();
@@ -501,27 +534,32 @@ __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice
   const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
   const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
   uint64_t cnt = 0, sb = 0, sp = 0;
-  bool cleared = false;
-  T bt[BPL], pt[SPL];
-  for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(d.work_counter + kOverflowTicket, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if (static_cast<uint64_t>(t) * 64 >= items) break;
-    const uint32_t i = t * 64 + lane;
-    const bool flagged = i < items && d.overflow[i] != 0;
-    uint64_t m = __builtin_amdgcn_ballot_w64(flagged);
-    if (flagged) d.overflow[i] = 0;
-    if (m && !cleared) {
-      WaveTable<K, K, CAP>{l.table_keys, l.table_keys + CAP}.clear();
-      wave_lds_sync();
-      cleared = true;
-    }
-    while (m) {
-      const uint32_t item = t * 64 + static_cast<uint32_t>(__builtin_ctzll(m));
-      m &= m - 1;
-      join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, l.table_keys, l.rb, l.rp, rel_b, rel_p, bt,
-                                                pt, cnt, sb, sp);
+  if (uniform_load(d.work_counter + kOverflowAny) != 0) {
+    bool cleared = false;
+    T bt[BPL], pt[SPL];
+    for (;;) {
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(d.work_counter + kOverflowTicket, 1u);
+      t = __builtin_amdgcn_readfirstlane(t);
+      if (static_cast<uint64_t>(t) * 64 * PER_LANE >= items) break;
+      for (uint32_t r = 0; r < PER_LANE; ++r) {
+        const uint32_t base = (t * PER_LANE + r) * 64;
+        const uint32_t i = base + lane;
+        const bool flagged = i < items && d.overflow[i] != 0;
+        uint64_t m = __builtin_amdgcn_ballot_w64(flagged);
+        if (flagged) d.overflow[i] = 0;
+        if (m && !cleared) {
+          WaveTable<K, K, CAP>{l.table_keys, l.table_keys + CAP}.clear();
+          wave_lds_sync();
+          cleared = true;
+        }
+        while (m) {
+          const uint32_t item = base + static_cast<uint32_t>(__builtin_ctzll(m));
+          m &= m - 1;
+          join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, l.table_keys, l.rb, l.rp, rel_b, rel_p,
+                                                    bt, pt, cnt, sb, sp);
+        }
+      }
     }
   }
   join_block_aggregate<WAVES>(d, cnt, sb, sp);
@@ -548,7 +586,9 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
   const WaveRuns& rb = l.rb;
   const WaveRuns& rp = l.rp;
   const uint32_t lane = lane_id();
+  static_assert(BPL * 64 <= kMarkBytes && SPL * 64 <= kMarkBytes, "run marks");
   WaveTable<K, K, CAP>{table_keys, table_keys + CAP}.clear();
+  for (uint32_t i = lane; i < 2 * kMarkBytes / 4; i += 64) reinterpret_cast<uint32_t*>(l.mb)[i] = 0;
   wave_lds_sync();
   const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
   const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
@@ -559,28 +599,38 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
   const T* const dummy = reinterpret_cast<const T*>(d.aggregates);  // 16 readable bytes, value unused
   const uint32_t per_wave = items / (gridDim.x * WAVES * 16);
   const uint32_t tk = per_wave < 1 ? 1 : (per_wave > 8 ? 8 : per_wave);
-  // ticket state: group share being drawn from, position in it
+  // Tickets: group g's share is a list of items; ticket t of it holds the tk
+  // items j0 + R * s (s < tk), j0 = (t / R) * R * tk + t % R, R = the waves
+  // drawing from one share.  So the R waves of a share work on R adjacent
+  // items at a time, and the cache lines two adjacent items' runs share are
+  // read once into that XCD's L2 rather than once per item.
+  constexpr uint32_t NG = XCD ? kJoinCounters : 1;
   const uint32_t g0 = XCD ? blockIdx.x % kJoinCounters : 0;
-  uint32_t gi = 0, grp = g0, it = 0, it_end = 0;
+  const uint32_t R = max(1u, gridDim.x * WAVES / NG);
+  uint32_t gi = 0, grp = g0, tbase = 0, ts = tk, tlen = 0;
   auto next_item = [&]() -> uint32_t {
-    while (it == it_end) {
-      if (gi >= (XCD ? kJoinCounters : 1u)) return items;
+    for (;;) {
+      if (ts < tk) {
+        const uint32_t j = tbase + R * ts++;
+        if (j < tlen) return XCD ? (grp + kJoinCounters * (j / F2)) * F2 + j % F2 : j;
+        ts = tk;  // later items of this ticket are past the share's end too
+      }
+      if (gi >= NG) return items;
       const uint32_t g = XCD ? (g0 + gi) % kJoinCounters : 0;
       const uint32_t len = XCD ? (g < F1 ? (F1 - g + kJoinCounters - 1) / kJoinCounters * F2 : 0) : items;
       uint32_t t = 0;
-      if (lane == 0 && len) t = atomicAdd(d.work_counter + g * kJoinCounterStride, tk);
+      if (lane == 0 && len) t = atomicAdd(d.work_counter + g * kJoinCounterStride, 1u);
       t = __builtin_amdgcn_readfirstlane(t);
-      if (len && t < len) {
+      const uint64_t j0 = static_cast<uint64_t>(t / R) * R * tk + t % R;
+      if (len && j0 < len) {
         grp = g;
-        it = t;
-        it_end = min(t + tk, len);
+        tbase = static_cast<uint32_t>(j0);
+        ts = 0;
+        tlen = len;
       } else {
         ++gi;
       }
     }
-    const uint32_t j = it++;
-    if (!XCD) return j;
-    return (grp + kJoinCounters * (j / F2)) * F2 + j % F2;
   };
   uint64_t cnt = 0, sb = 0, sp = 0;
   uint32_t nb = 0, np = 0;
@@ -615,18 +665,17 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
 #pragma unroll
     for (uint32_t k = 0; k < SPL; ++k) ip[k] = 0;
     if (live) {
-      tb = publish_runs(finish_run(d, xb, 0, nb), rb);
-      tp = publish_runs(finish_run(d, xp, 0, np), rp);
+      const LaneRun lb = finish_run(d, xb, 0, nb), lp = finish_run(d, xp, 0, np);
+      tb = publish_runs(lb, rb);
+      tp = publish_runs(lp, rp);
       wave_lds_sync();
       nxt = next_item();
       const bool empty = (tb == 0 && nb <= kMaxRuns) || (tp == 0 && np <= kMaxRuns);  // tb, tp: first 64 runs
       const bool fast = nb <= kMaxRuns && np <= kMaxRuns && tb <= FAST_B && tp <= SPL * 64;
       if (!empty && fast) {
-        // gather indices from LDS (lanes past the end repeat the last tuple)
-#pragma unroll
-        for (uint32_t k = 0; k < BPL; ++k) ib[k] = batch_element(rb, min(k * 64 + lane, tb - 1));
-#pragma unroll
-        for (uint32_t k = 0; k < SPL; ++k) ip[k] = batch_element(rp, min(k * 64 + lane, tp - 1));
+        // gather indices (lanes past the end repeat the last tuple)
+        run_indices<BPL>(rb, l.mb, lb, tb, ib);
+        run_indices<SPL>(rp, l.mp, lp, tp, ip);
         gb = rel_b;
         gp = rel_p;
         staged = true;
@@ -634,6 +683,7 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
         // does not fit the registers: deferred to join_general_kernel (an
         // empty item has no match)
         d.overflow[cur] = 1;
+        d.work_counter[kOverflowAny] = 1;
       }
     }
     // Every step issues the same gathers back to back (a step that stages
@@ -707,30 +757,24 @@ size_t pipeline_sort2_lds(KeyType t, uint32_t F2) {
 }
 
 template <typename K, uint32_t CAP, uint32_t WAVES>
-hipError_t launch_join_t(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
-  constexpr uint32_t PL = sizeof(K) == 8 ? 9 : 10;  // tuples per lane and side (register budget)
+hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, hipStream_t stream) {
+  constexpr uint32_t PL = 10;  // tuples per lane and side: two buffers fit 2 waves per SIMD
   const size_t lds = join_lds_bytes<K, CAP, WAVES>();
-  auto kern = join_kernel<K, CAP, WAVES, PL, PL, true, true>;
+  auto kern = deferred ? join_general_kernel<K, CAP, WAVES, 12, 8> : join_kernel<K, CAP, WAVES, PL, PL, true, true>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  auto gen = join_general_kernel<K, CAP, WAVES, 12, 8>;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(gen), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(lds));
-  hipLaunchKernelGGL(gen, dim3(grid), dim3(WAVES * 64), lds, stream, d);
   return hipGetLastError();
 }
 
-hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, uint32_t grid,
-                                 hipStream_t stream) {
+hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, bool deferred, uint32_t grid,
+                                hipStream_t stream) {
   if (t == KeyType::I32) {
-    if (large_tables) return launch_join_t<int32_t, 8192, 2>(d, grid, stream);
-    return launch_join_t<int32_t, 1024, 4>(d, grid, stream);
+    if (large_tables) return launch_join_t<int32_t, 8192, 2>(d, deferred, grid, stream);
+    return launch_join_t<int32_t, 1024, 4>(d, deferred, grid, stream);
   }
-  if (large_tables) return launch_join_t<int64_t, 4096, 2>(d, grid, stream);
-  return launch_join_t<int64_t, 1024, 4>(d, grid, stream);
+  if (large_tables) return launch_join_t<int64_t, 4096, 2>(d, deferred, grid, stream);
+  return launch_join_t<int64_t, 1024, 4>(d, deferred, grid, stream);
 }
 
 }  // namespace fij

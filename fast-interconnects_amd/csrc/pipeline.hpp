@@ -34,11 +34,13 @@ struct PipelineSide {
 
 // Join work counters, each on its own 128-byte line: one per XCD (work
 // groups are dispatched to the 8 XCDs round-robin), then the ticket counter
-// of the kernel that joins the deferred items.
+// of the kernel that joins the deferred items, then a word that is non-zero
+// when any item was deferred.
 constexpr uint32_t kJoinCounters = 8;
 constexpr uint32_t kJoinCounterStride = 32;
 constexpr uint32_t kOverflowTicket = kJoinCounters * kJoinCounterStride;
-constexpr uint32_t kJoinCounterWords = (kJoinCounters + 1) * kJoinCounterStride;
+constexpr uint32_t kOverflowAny = (kJoinCounters + 1) * kJoinCounterStride;
+constexpr uint32_t kJoinCounterWords = (kJoinCounters + 2) * kJoinCounterStride;
 
 struct PipelineDevice {
   PipelineSide build;
@@ -58,7 +60,9 @@ size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
 // Pass 2 of both relations in one persistent launch of `grid` blocks.
 hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t grid, hipStream_t stream);
-hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, uint32_t grid,
+// The register-resident join of every work item (deferred = false), then
+// the join of the items it deferred (deferred = true).
+hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, bool deferred, uint32_t grid,
                                 hipStream_t stream);
 
 }  // namespace fij

@@ -33,7 +33,7 @@ struct fij_join_plan {
   uint32_t lds_total;
   bool large_tables;
   std::vector<void*> allocs;
-  hipEvent_t ev[5];
+  hipEvent_t ev[6];
   JoinAggregate* host_agg;
 };
 
@@ -176,7 +176,7 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
   if (st == FIJ_OK && (hipMemset(p->d.overflow, 0, items) != hipSuccess || hipDeviceSynchronize() != hipSuccess))
     st = plan_fail(FIJ_DEVICE_ERROR, "clearing the deferred-item flags failed");
   if (st == FIJ_OK) st = alloc(p, &p->d.aggregates, p->join_grid);
-  for (int i = 0; i < 5 && st == FIJ_OK; ++i)
+  for (int i = 0; i < 6 && st == FIJ_OK; ++i)
     if (hipEventCreate(&p->ev[i]) != hipSuccess) st = plan_fail(FIJ_DEVICE_ERROR, "hipEventCreate failed");
   if (st == FIJ_OK && hipHostMalloc(reinterpret_cast<void**>(&p->host_agg), sizeof(JoinAggregate) * p->join_grid,
                                     hipHostMallocDefault) != hipSuccess)
@@ -239,8 +239,10 @@ fij_status fij_join_plan_enqueue(fij_join_plan* p, const void* build_keys, const
   PLAN_HIP(launch_pipeline_plan(d, st));
   if (d.bits2) PLAN_HIP(launch_pipeline_sort2(d, p->kt, p->sort2_grid, st));
   PLAN_HIP(hipEventRecord(p->ev[3], st));
-  PLAN_HIP(launch_pipeline_join(d, p->kt, p->large_tables, p->join_grid, st));
+  PLAN_HIP(launch_pipeline_join(d, p->kt, p->large_tables, false, p->join_grid, st));
   PLAN_HIP(hipEventRecord(p->ev[4], st));
+  PLAN_HIP(launch_pipeline_join(d, p->kt, p->large_tables, true, p->join_grid, st));
+  PLAN_HIP(hipEventRecord(p->ev[5], st));
   PLAN_HIP(hipMemcpyAsync(p->host_agg, d.aggregates, sizeof(JoinAggregate) * p->join_grid, hipMemcpyDeviceToHost,
                           st));
   return FIJ_OK;
@@ -261,19 +263,18 @@ fij_status fij_join_plan_result(fij_join_plan* p, void* stream, fij_join_aggrega
     result->sum_probe = s;
   }
   if (timing) {
-    float a = 0, b = 0, c = 0, t = 0, c2 = 0, c3 = 0;
-    PLAN_HIP(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
-    PLAN_HIP(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
-    PLAN_HIP(hipEventElapsedTime(&c, p->ev[2], p->ev[4]));
-    PLAN_HIP(hipEventElapsedTime(&t, p->ev[0], p->ev[4]));
-    PLAN_HIP(hipEventElapsedTime(&c2, p->ev[2], p->ev[3]));
-    PLAN_HIP(hipEventElapsedTime(&c3, p->ev[3], p->ev[4]));
-    timing->prefix_sum_ms = a;
-    timing->partition_ms = b;
-    timing->join_ms = c;
-    timing->total_ms = t;
-    timing->pass2_ms = c2;
-    timing->build_probe_ms = c3;
+    float ms[6] = {0, 0, 0, 0, 0, 0};
+    const int from[6] = {0, 1, 2, 0, 2, 3}, to[6] = {1, 2, 5, 5, 3, 4};
+    for (int i = 0; i < 6; ++i) PLAN_HIP(hipEventElapsedTime(&ms[i], p->ev[from[i]], p->ev[to[i]]));
+    float deferred = 0;
+    PLAN_HIP(hipEventElapsedTime(&deferred, p->ev[4], p->ev[5]));
+    timing->prefix_sum_ms = ms[0];
+    timing->partition_ms = ms[1];
+    timing->join_ms = ms[2];
+    timing->total_ms = ms[3];
+    timing->pass2_ms = ms[4];
+    timing->build_probe_ms = ms[5];
+    timing->deferred_ms = deferred;
   }
   return FIJ_OK;
 }

@@ -721,7 +721,8 @@ class _PlanConfig(C.Structure):
 
 class _Timing(C.Structure):
     _fields_ = [("prefix_sum_ms", C.c_float), ("partition_ms", C.c_float), ("join_ms", C.c_float),
-                ("total_ms", C.c_float), ("pass2_ms", C.c_float), ("build_probe_ms", C.c_float)]
+                ("total_ms", C.c_float), ("pass2_ms", C.c_float), ("build_probe_ms", C.c_float),
+                ("deferred_ms", C.c_float)]
 
 
 _proto("fij_join_plan_create", C.c_int, C.POINTER(_PlanConfig), C.POINTER(_vp))
@@ -770,7 +771,8 @@ class JoinPlan:
         check(lib.fij_join_plan_result(self.h, _stream(stream), C.byref(r), C.byref(t)))
         return (r.count, r.sum_build, r.sum_probe), {
             "prefix_sum_ms": t.prefix_sum_ms, "partition_ms": t.partition_ms, "join_ms": t.join_ms,
-            "total_ms": t.total_ms, "pass2_ms": t.pass2_ms, "build_probe_ms": t.build_probe_ms}
+            "total_ms": t.total_ms, "pass2_ms": t.pass2_ms, "build_probe_ms": t.build_probe_ms,
+            "deferred_ms": t.deferred_ms}
 
     def execute(self, bk, bp, sk, sp, stream=None):
         self.enqueue(bk, bp, sk, sp, stream)

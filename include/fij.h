@@ -312,8 +312,9 @@ typedef struct fij_join_plan_config {
 /* Phase times as in the reference's RadixJoinPoint
  * (radix-join/src/measurement/data_point.rs:29-74): prefix sum (pass-1
  * histograms), partition (pass-1 scatter), join (pass 2 + build/probe);
- * join_ms splits into pass2_ms (plan + pass-2 sort) and build_probe_ms (the
- * build/probe launch alone). */
+ * join_ms splits into pass2_ms (plan + pass-2 sort), build_probe_ms (the
+ * register-resident build/probe launch) and deferred_ms (the launch that
+ * joins the partitions too large for it). */
 typedef struct fij_join_timing {
   float prefix_sum_ms;
   float partition_ms;
@@ -321,6 +322,7 @@ typedef struct fij_join_timing {
   float total_ms;
   float pass2_ms;
   float build_probe_ms;
+  float deferred_ms;
 } fij_join_timing;
 
 fij_status fij_join_plan_create(const fij_join_plan_config *config, fij_join_plan **out);
