@@ -44,6 +44,20 @@ def parse():
     ap.add_argument("--cpu-log2n", type=int, default=23, help="CPU baseline sample size (log2 of |R| = |S|)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--verify", action="store_true", help="check the result against the oracle at full size")
+    ap.add_argument("--workload", choices=["plain", "skew", "batched", "served"], default="plain",
+                    help="plain: R = S = 2^27 int64, 9+9 bits (the headline line); skew: R = 2^27 int32 unique, "
+                         "S = 2^30 int32 Zipf over R, 8+8 bits; batched: 4096 int32 joins of log-uniform sizes")
+    ap.add_argument("--zipf", type=float, default=1.0, help="skew: Zipf exponent")
+    ap.add_argument("--skew-log2s", type=int, default=30, help="skew: log2 |S|")
+    ap.add_argument("--joins", type=int, default=4096, help="batched: number of joins")
+    ap.add_argument("--wave-gib", type=float, default=24.0, help="batched: input bytes per submission wave")
+    ap.add_argument("--builds", type=int, default=256, help="served: resident build sides")
+    ap.add_argument("--probe-inputs", type=int, default=48, help="served: distinct probe sides in device memory")
+    ap.add_argument("--requests", type=int, default=2000, help="served: requests in the Poisson stream")
+    ap.add_argument("--load", type=float, default=0.8, help="served: offered load as a fraction of capacity")
+    ap.add_argument("--hbm-gb", type=float, default=200.0, help="served: HBM budget (GB)")
+    ap.add_argument("--plain-sample", type=int, default=0,
+                    help="batched: time the plain per-join launch on every k-th join only (0 = all)")
     return ap.parse_args()
 
 
@@ -76,6 +90,24 @@ def cpu_baseline(log2n, bits1, bits2, threads):
     }
 
 
+def measured_traffic(kernel, log2n, bits1, bits2):
+    """HBM bytes per launch of `kernel` at this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter values scaled by what
+    the same counters read for the plain copy kernel); None when the summary
+    does not cover this workload or kernel."""
+    path = os.path.join(REPO, "profiles", "radix_join_2e27_i64_9+9_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if (w.get("log2n"), w.get("bits1"), w.get("bits2")) != (log2n, bits1, bits2):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["fetch_bytes"] + k["write_bytes"]
+
+
 def stream_copy_gbs(fij, nbytes, reps=5):
     """HBM bandwidth of the library's plain copy kernel (bytes read + written
     per second): what a streaming kernel reaches on this GPU."""
@@ -92,12 +124,302 @@ def stream_copy_gbs(fij, nbytes, reps=5):
     return 2 * nbytes / (ms / 1000.0) / 1e9
 
 
+def skew_main(a):
+    """Skewed join: R = 2^27 int32 tuples with unique permuted keys 1..2^27,
+    S = 2^skew_log2s int32 tuples with Zipf(--zipf) keys over R's keys
+    (payload = row index), 8 + 8 radix bits.  Heavy (pass-1, pass-2)
+    partitions are split into work units joined by many waves."""
+    import fij
+
+    n_r, n_s = 1 << 27, 1 << a.skew_log2s
+    kt = fij.KeyType.I32
+    rk = fij.DeviceArray(n_r, np.int32)
+    rp = fij.DeviceArray(n_r, np.int32)
+    sk = fij.DeviceArray(n_s, np.int32)
+    sp = fij.DeviceArray(n_s, np.int32)
+    fij.gen_permutation(rk, SEED, 1)
+    fij.gen_iota(rp, 1)
+    fij.gen_zipf(sk, n_r, a.zipf, SEED, 1, SEED, 1)
+    fij.gen_iota(sp, 0)
+    fij.synchronize()
+    plan = fij.JoinPlan(kt, n_r, n_s, 8, 8)
+    for _ in range(a.warmup):
+        res, _ = plan.execute(rk, rp, sk, sp)
+    times, phases = [], []
+    fij.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res, t = plan.execute(rk, rp, sk, sp)
+        phases.append(t)
+    wall = time.perf_counter() - t0
+    ms = wall * 1000.0 / a.steps
+    assert res[0] == n_s, f"match count {res[0]} != {n_s}"  # every S key is one of R's keys
+    # closed forms: R.value = index + 1 of the key's row; S.value = row index
+    assert res[2] == (n_s * (n_s - 1) // 2) % (1 << 64)
+    gpu_ms = float(np.median([t["total_ms"] for t in phases]))
+    b_alg = (n_r + n_s) * (4 + 5 * 8)
+    out = {
+        "metric": METRIC, "value": (n_r + n_s) / (ms / 1000.0), "unit": "tuples/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded on-device generators, Zipf probe keys)",
+        "config": {"workload": f"skewed radix join, R = 2^27 int32 unique keys, S = 2^{a.skew_log2s} int32 "
+                               f"Zipf({a.zipf}) over R's keys, 8+8 radix bits, count + sums",
+                   "build_tuples": n_r, "probe_tuples": n_s, "radix_bits": [8, 8], "zipf": a.zipf,
+                   "parallelism": "single GPU"},
+        "gpu_ms_per_join": gpu_ms,
+        "phases_ms": {k: float(np.median([t[k] for t in phases])) for k in phases[0]},
+        "pipeline_roofline_frac": b_alg / (gpu_ms / 1000.0) / (HBM_PEAK_GBS * 1e9),
+        "result": {"count": res[0], "sum_build": res[1], "sum_probe": res[2]},
+    }
+    print(json.dumps(out))
+
+
+def batched_main(a):
+    """Batched joins: a.joins independent int32 joins; |R| log-uniform in
+    2^10..2^22 with unique permuted keys, |S| = |R| x U{4..16} with keys
+    uniform over that join's R (payloads: row indices).  The stream is
+    generated and joined in waves of at most --wave-gib of input.  Measured:
+    the batched form (one submission per wave), and the plain per-join launch
+    (a plan created, executed and destroyed per join, as one run of the
+    reference's radix-join binary per pair) and the same with plans reused
+    per size class (one synchronous join at a time)."""
+    import fij
+
+    rng = np.random.default_rng(SEED)
+    n_r = np.floor(2.0 ** rng.uniform(10, 22, a.joins)).astype(np.int64)
+    n_s = n_r * rng.integers(4, 17, a.joins)
+    total = int((n_r + n_s).sum())
+    per_join_bytes = (n_r + n_s) * 8
+    waves, cur, acc = [], [], 0
+    for i in range(a.joins):
+        if cur and acc + per_join_bytes[i] > a.wave_gib * 2**30:
+            waves.append(cur)
+            cur, acc = [], 0
+        cur.append(i)
+        acc += per_join_bytes[i]
+    waves.append(cur)
+    batch = fij.JoinBatch(np.int32, max(len(w) for w in waves), use_graph=True)
+    res_dev = fij.DeviceArray(3, np.uint64)
+    pooled = {}
+    t_batch = t_plain = t_reused = 0.0
+    gpu_batch = 0.0
+    stats_acc = {"small_joins": 0, "partitioned_joins": 0, "prepare_ms": 0.0}
+    checked = 0
+    for w in waves:
+        # inputs of the wave in one arena
+        lens = [(int(n_r[i]), int(n_s[i])) for i in w]
+        words = sum(2 * (r + s) for r, s in lens)
+        arena = fij.DeviceArray(words, np.int32)
+        joins, off = [], 0
+        for i, (r, s_) in zip(w, lens):
+            rk = arena.view(off, off + r); off += r
+            rp = arena.view(off, off + r); off += r
+            sk = arena.view(off, off + s_); off += s_
+            sp = arena.view(off, off + s_); off += s_
+            fij.gen_permutation(rk, SEED + i, 1)
+            fij.gen_iota(rp, 1)
+            fij.gen_uniform(sk, SEED + i, 1, 1, r)
+            fij.gen_iota(sp, 0)
+            joins.append((rk, rp, sk, sp))
+        fij.synchronize()
+        batch.execute(joins[: min(8, len(joins))])  # warm the kernels
+        t0 = time.perf_counter()
+        got, st = batch.execute(joins)
+        t_batch += time.perf_counter() - t0
+        gpu_batch += st["gpu_ms"] / 1000.0
+        for k in ("small_joins", "partitioned_joins", "prepare_ms"):
+            stats_acc[k] += st[k]
+        for (r, s_), g in zip(lens, got):
+            assert g[0] == s_ and g[2] == s_ * (s_ - 1) // 2, "batched join result"
+        # the plain per-join launch (every k-th join when sampling)
+        step = a.plain_sample or 1
+        for j in range(0, len(joins), step):
+            r, s_ = lens[j]
+            small, b1, b2 = fij.autotune(np.int32, r, s_)
+            t0 = time.perf_counter()
+            plan = fij.JoinPlan(fij.KeyType.I32, r, s_, b1, b2)
+            g, _ = plan.execute(*joins[j])
+            del plan
+            fij.synchronize()
+            t_plain += (time.perf_counter() - t0) * step
+            assert g == got[j], "plain and batched results differ"
+            key = (b1, b2)
+            cap = pooled.get(key, (None, 0, 0))
+            if cap[1] < r or cap[2] < s_:  # a plan per size class, grown to the largest join seen
+                pooled[key] = (fij.JoinPlan(fij.KeyType.I32, max(r, cap[1]), max(s_, cap[2]), b1, b2),
+                               max(r, cap[1]), max(s_, cap[2]))
+            t0 = time.perf_counter()
+            pooled[key][0].enqueue_sized(*joins[j], res_dev)
+            fij.synchronize()
+            t_reused += (time.perf_counter() - t0) * step
+            checked += 1
+        del joins, arena
+    value = total / t_batch
+    out = {
+        "metric": METRIC, "value": value, "unit": "tuples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_batch * 1000.0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic (seeded on-device generators)",
+        "config": {"workload": f"batched joins: {a.joins} int32 joins, |R| log-uniform 2^10..2^22, "
+                               "|S| = |R| x U{4..16}, count + sums per join", "joins": a.joins,
+                   "tuples": total, "waves": len(waves), "parallelism": "single GPU"},
+        "batched": {"seconds": t_batch, "gpu_seconds": gpu_batch, "tuples_per_s": value, **stats_acc},
+        "plain_per_join": {"seconds": t_plain, "tuples_per_s": total / t_plain,
+                           "sampled_every": a.plain_sample or 1},
+        "plain_reused_plans": {"seconds": t_reused, "tuples_per_s": total / t_reused},
+        "speedup_vs_plain": t_plain / t_batch,
+        "speedup_vs_reused_plans": t_reused / t_batch,
+    }
+    print(json.dumps(out))
+
+
+def served_main(a):
+    """Served joins: a Poisson stream of requests against a pool of a.builds
+    int64 build sides (|R| log-uniform 2^12..2^26, keys a permutation of
+    1..|R|, payload = key), partitioned once and kept resident.  Each request
+    brings one of a.probe_inputs probe sides (|S| log-uniform 2^14..2^28,
+    keys uniform over 1..1.25|R|: 80% join, 20% miss; payload = row index)
+    held in device memory.  Slots and resident builds are charged against
+    the HBM budget.  Measured: capacity (all requests queued at once), then
+    latency percentiles and throughput at a.load x capacity, and the plain
+    per-join launch (a plan created, executed and destroyed per join, both
+    sides partitioned, as one run of the reference's radix-join binary) over
+    the same stream.  Every result equals the same request joined alone."""
+    import threading
+
+    import fij
+
+    rng = np.random.default_rng(SEED)
+    n_b = np.floor(2.0 ** rng.uniform(12, 26, a.builds)).astype(np.int64)
+    n_p = np.floor(2.0 ** rng.uniform(14, 28, a.probe_inputs)).astype(np.int64)
+    which = rng.integers(0, a.builds, a.probe_inputs)
+    slot_classes = [(8, 1 << 20), (4, 1 << 24), (2, 1 << 28)]
+    srv = fij.JoinServer(np.int64, slot_classes, max_probe_len=1 << 28, hbm_budget=int(a.hbm_gb * 1e9))
+    scratch_k = fij.DeviceArray(int(n_b.max()), np.int64)
+    scratch_p = fij.DeviceArray(int(n_b.max()), np.int64)
+    t0 = time.perf_counter()
+    ids = []
+    for i, n in enumerate(n_b):
+        k, p = scratch_k.view(0, int(n)), scratch_p.view(0, int(n))
+        fij.gen_permutation(k, SEED + i, 1)
+        fij.check(fij.lib.fij_memcpy(fij.C.c_void_p(p.ptr), fij.C.c_void_p(k.ptr), k.nbytes(), None))
+        ids.append(srv.add_build(k, p))
+    t_register = time.perf_counter() - t0
+    del scratch_k, scratch_p
+    probes = []
+    for j in range(a.probe_inputs):
+        b = int(which[j])
+        sk = fij.DeviceArray(int(n_p[j]), np.int64)
+        sp = fij.DeviceArray(int(n_p[j]), np.int64)
+        fij.gen_uniform(sk, SEED + 7 * j, 3, 1, int(n_b[b]) * 5 // 4)
+        fij.gen_iota(sp, 0)
+        probes.append((b, sk, sp))
+    fij.synchronize()
+    used, _ = srv.memory()
+    # each probe input joined alone, and by the plain per-join launch
+    alone, t_plain = [], []
+    bk = fij.DeviceArray(int(n_b.max()), np.int64)
+    for j, (b, sk, sp) in enumerate(probes):
+        alone.append(srv.wait(srv.submit(ids[b], sk, sp))[0])
+        n = int(n_b[b])
+        k, p = bk.view(0, n), bk.view(0, n)  # payload = key
+        fij.gen_permutation(k, SEED + b, 1)
+        fij.synchronize()
+        _, b1, b2 = fij.autotune(np.int64, n, int(n_p[j]))
+        t0 = time.perf_counter()
+        plan = fij.JoinPlan(fij.KeyType.I64, n, int(n_p[j]), b1, b2)
+        g, _ = plan.execute(k, p, sk, sp)
+        del plan
+        fij.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+        assert g == alone[j], f"probe input {j}: served {alone[j]} != plain {g}"
+    del bk
+    stream = rng.integers(0, a.probe_inputs, a.requests)
+    tuples = np.array([int(n_b[probes[j][0]]) + int(n_p[j]) for j in range(a.probe_inputs)], dtype=np.float64)
+
+    def run(arrivals):
+        rids = [None] * len(stream)
+        lat = np.zeros(len(stream))
+        done = threading.Event()
+        errors = []
+
+        def waiter():
+            try:
+                for i in range(len(stream)):
+                    while rids[i] is None:
+                        time.sleep(20e-6)
+                    got, t = srv.wait(rids[i])
+                    lat[i] = t["latency_ms"]
+                    if got != alone[stream[i]]:
+                        errors.append(i)
+            finally:
+                done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        for i, j in enumerate(stream):
+            if arrivals is not None:
+                dt = arrivals[i] - (time.perf_counter() - start)
+                if dt > 300e-6:
+                    time.sleep(dt - 200e-6)
+                while time.perf_counter() - start < arrivals[i]:
+                    pass
+            rids[i] = srv.submit(ids[probes[j][0]], probes[j][1], probes[j][2])
+        th.join()
+        wall = time.perf_counter() - start
+        assert not errors, f"requests {errors[:5]} differ from the same request alone"
+        return wall, lat
+
+    run(None)  # warm-up
+    wall_sat, lat_sat = run(None)
+    capacity = len(stream) / wall_sat
+    rate = a.load * capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(stream)))
+    wall, lat = run(arrivals)
+    total = float(tuples[stream].sum())
+    # per request: the probe side's histogram, two passes and join read
+    # (k + 5t per tuple) plus the join's read of the resident build side (t)
+    b_alg = float(sum(int(n_p[j]) * 88 + int(n_b[probes[j][0]]) * 16 for j in stream))
+    plain_s = float(np.array(t_plain)[stream].sum())
+    value = total / wall
+    out = {
+        "metric": METRIC, "value": value, "unit": "tuples/s", "n_gpus": 1, "steps": len(stream), "warmup": 1,
+        "ms_per_step": wall * 1000.0 / len(stream), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic (seeded on-device generators)",
+        "config": {"workload": f"served joins: Poisson stream of {len(stream)} requests at {a.load} x capacity, "
+                               f"{a.builds} resident int64 builds |R| log-uniform 2^12..2^26, probes |S| "
+                               "log-uniform 2^14..2^28 (80% matching keys), count + sums per request",
+                   "builds": a.builds, "probe_inputs": a.probe_inputs, "requests": len(stream),
+                   "slot_classes": slot_classes, "hbm_budget_gb": a.hbm_gb, "parallelism": "single GPU"},
+        "tuples_note": "value counts |R| + |S| per request (the joined relations); the resident |R| is "
+                       "partitioned once at registration",
+        "probe_tuples_per_s": float(n_p[stream].sum()) / wall,
+        "pipeline_roofline_frac": b_alg / wall / (HBM_PEAK_GBS * 1e9),
+        "offered_rate_per_s": rate, "capacity_per_s": capacity,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "saturated": {"tuples_per_s": total / wall_sat, "p50_ms": float(np.percentile(lat_sat, 50)),
+                      "p99_ms": float(np.percentile(lat_sat, 99))},
+        "plain_per_join": {"seconds": plain_s, "tuples_per_s": total / plain_s},
+        "speedup_vs_plain": (total / wall) / (total / plain_s),
+        "speedup_vs_plain_saturated": plain_s / wall_sat,
+        "registration_s": t_register, "server_device_bytes": used,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
               "(the radix join runs on one MI355X, multi-GPU sharding is design only)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "skew":
+        return skew_main(a)
+    if a.workload == "batched":
+        return batched_main(a)
+    if a.workload == "served":
+        return served_main(a)
     import fij
 
     n = 1 << a.log2n
@@ -183,7 +505,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dominant, a.log2n, a.bits1, a.bits2),
         },
         "result": {"count": res[0], "sum_build": res[1], "sum_probe": res[2]},
     }

@@ -729,6 +729,16 @@ fij_status fij_gen_uniform(fij_type type, void* out, uint64_t n, uint64_t seed, 
   return FIJ_OK;
 }
 
+fij_status fij_gen_zipf(fij_type type, void* out, uint64_t n, uint64_t num_ranks, double exponent, uint64_t seed,
+                        uint64_t stream_id, uint64_t perm_seed, int64_t base, void* stream) {
+  if (!valid_type(type) || (n && !out) || num_ranks == 0 || !(exponent >= 0.0))
+    return fail(FIJ_INVALID_ARGUMENT, "invalid argument");
+  if (n == 0) return FIJ_OK;
+  FIJ_HIP(launch_gen_zipf(key_type(type), out, n, num_ranks, exponent, seed, stream_id, perm_seed, base,
+                          static_cast<hipStream_t>(stream)));
+  return FIJ_OK;
+}
+
 fij_status fij_gen_iota(fij_type type, void* out, uint64_t n, int64_t base, void* stream) {
   if (!valid_type(type) || (n && !out)) return fail(FIJ_INVALID_ARGUMENT, "invalid argument");
   if (n == 0) return FIJ_OK;

@@ -64,6 +64,61 @@ __global__ void gen_perm_kernel(K* out, uint64_t n, uint64_t seed, int64_t base)
     out[i] = static_cast<K>(base + static_cast<int64_t>(feistel_perm(i, n, seed, h, mask, rk)));
 }
 
+// Zipf(s) over ranks 1..N by rejection-inversion (W. Hormann and G.
+// Derflinger, "Rejection-inversion to generate variates from monotone
+// discrete distributions", ACM TOMACS 6(3), 1996); H is the antiderivative of
+// h(x) = x^-s written with log1p / expm1 forms for stability near s = 1.
+// Attempt a of element i draws u from rand_u64(seed, stream, 64 i + a).
+__device__ __forceinline__ double zh1(double x) {
+  return fabs(x) > 1e-8 ? log1p(x) / x : 1.0 - x * (0.5 - x * (1.0 / 3.0 - 0.25 * x));
+}
+__device__ __forceinline__ double zh2(double x) {
+  return fabs(x) > 1e-8 ? expm1(x) / x : 1.0 + x * 0.5 * (1.0 + x * (1.0 / 3.0) * (1.0 + 0.25 * x));
+}
+__device__ __forceinline__ double zipf_H(double x, double s) {
+  const double lx = log(x);
+  return zh2((1.0 - s) * lx) * lx;
+}
+__device__ __forceinline__ double zipf_Hinv(double y, double s) {
+  double t = y * (1.0 - s);
+  if (t < -1.0) t = -1.0;
+  return exp(zh1(t) * y);
+}
+__device__ __forceinline__ double zipf_h(double x, double s) { return exp(-s * log(x)); }
+
+// Keys base + perm(rank - 1): the keys gen_perm_kernel makes for N keys with
+// perm_seed, with Zipf frequencies (rank 1 the most frequent).
+template <typename K>
+__global__ void gen_zipf_kernel(K* out, uint64_t n, uint64_t N, double s, uint64_t seed, uint64_t stream_id,
+                                uint64_t perm_seed, int64_t base) {
+  const double Nd = static_cast<double>(N);
+  const double hx1 = zipf_H(1.5, s) - 1.0;
+  const double hN = zipf_H(Nd + 0.5, s);
+  const double sc = 2.0 - zipf_Hinv(zipf_H(2.5, s) - zipf_h(2.0, s), s);
+  const uint32_t b = N > 1 ? 64u - __builtin_clzll(N - 1) : 1u;
+  const uint32_t h = (b + 1u) / 2u;
+  const uint64_t mask = (1ull << h) - 1ull;
+  uint64_t rk[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rk[r] = mix64(perm_seed + (r + 1) * kGolden);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t k = 1;
+    for (uint64_t a = 0; a < 64; ++a) {
+      const double u01 = static_cast<double>(rand_u64(seed, stream_id, i * 64 + a) >> 11) * 0x1.0p-53;
+      const double u = hN + u01 * (hx1 - hN);
+      const double x = zipf_Hinv(u, s);
+      double kd = floor(x + 0.5);
+      if (kd < 1.0) kd = 1.0;
+      if (kd > Nd) kd = Nd;
+      k = static_cast<uint64_t>(kd);
+      if (kd - x <= sc || u >= zipf_H(kd + 0.5, s) - zipf_h(kd, s)) break;
+    }
+    const uint64_t pos = N > 1 ? feistel_perm(k - 1, N, perm_seed, h, mask, rk) : 0;
+    out[i] = static_cast<K>(base + static_cast<int64_t>(pos));
+  }
+}
+
 template <typename K>
 __global__ void gen_uniform_kernel(K* out, uint64_t n, uint64_t seed, uint64_t stream_id, int64_t lo,
                                    uint64_t range) {
@@ -130,6 +185,17 @@ hipError_t launch_gen_uniform(KeyType t, void* out, uint64_t n, uint64_t seed, u
   else
     hipLaunchKernelGGL(gen_uniform_kernel<int64_t>, dim3(grid_for(n)), dim3(256), 0, stream,
                        static_cast<int64_t*>(out), n, seed, stream_id, lo, range);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_zipf(KeyType t, void* out, uint64_t n, uint64_t num_ranks, double exponent, uint64_t seed,
+                           uint64_t stream_id, uint64_t perm_seed, int64_t base, hipStream_t stream) {
+  if (t == KeyType::I32)
+    hipLaunchKernelGGL(gen_zipf_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream, static_cast<int32_t*>(out),
+                       n, num_ranks, exponent, seed, stream_id, perm_seed, base);
+  else
+    hipLaunchKernelGGL(gen_zipf_kernel<int64_t>, dim3(grid_for(n)), dim3(256), 0, stream, static_cast<int64_t*>(out),
+                       n, num_ranks, exponent, seed, stream_id, perm_seed, base);
   return hipGetLastError();
 }
 

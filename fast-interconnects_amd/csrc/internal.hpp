@@ -123,6 +123,8 @@ hipError_t launch_gen_perm_keys(KeyType t, void* out, uint64_t n, uint64_t seed,
 hipError_t launch_gen_uniform(KeyType t, void* out, uint64_t n, uint64_t seed, uint64_t stream_id,
                               int64_t lo, uint64_t range, hipStream_t stream);
 hipError_t launch_gen_iota(KeyType t, void* out, uint64_t n, int64_t base, hipStream_t stream);
+hipError_t launch_gen_zipf(KeyType t, void* out, uint64_t n, uint64_t num_ranks, double exponent, uint64_t seed,
+                           uint64_t stream_id, uint64_t perm_seed, int64_t base, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, uint64_t words16, hipStream_t stream);

@@ -34,14 +34,14 @@ namespace {
 // --------------------------------------------------------------- plan ----
 
 // Partition lengths, starts in the concatenated pass-2 output and the tile
-// map (tile_start[p] = first tile of p) of one relation per block
-// (blockIdx.x selects it); block 0 also resets the join's work counters and
+// map (tile_start[p] = first tile of p) of one relation per block (the sides
+// in d.sides, build first); block 0 also resets the join's work counters and
 // aggregates.  Pass-1 partition p of the contiguous layout is the range
 // [off1[p], off1[p + 1] - pad), the last one ends at padded_len1.
 __global__ __launch_bounds__(1024) void plan_kernel(PipelineDevice d) {
   extern __shared__ uint64_t buf[];
   __shared__ uint64_t wave_tmp[32];
-  const uint32_t rel = blockIdx.x;
+  const uint32_t rel = d.sides == 2 ? 1u : blockIdx.x;
   const PipelineSide& s = rel == 0 ? d.build : d.probe;
   const uint32_t F1 = d.F1;
   uint64_t* starts = buf;              // [F1]
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(1024) void plan_kernel(PipelineDevice d) {
     s.pstart[F1] = total;
     s.tile_start[F1] = static_cast<uint32_t>(total_tiles);
   }
-  if (rel == 0) {
+  if (blockIdx.x == 0) {
     if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
     for (uint32_t i = threadIdx.x; i < d.agg_len; i += blockDim.x) {
       d.aggregates[i].count = 0;
@@ -93,8 +93,9 @@ struct TileRef {
   const Tuple<K>* src;
 };
 
-// Pass 2 of both relations as one persistent launch: block b sorts tiles b,
-// b + grid, ... of the concatenated tile lists (build tiles first).  A tile
+// Pass 2 of both relations (the sides in d.sides) as one persistent launch:
+// block b sorts tiles b, b + grid, ... of the concatenated tile lists (build
+// tiles first).  A tile
 // is an LDS counting sort of up to TILE2 tuples of one pass-1 partition by
 // the pass-2 digit: one coalesced read, one coalesced write, and per digit
 // its run bounds in meta.  The next tile's tuples are loaded while the
@@ -108,8 +109,8 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
   uint32_t* cnt = reinterpret_cast<uint32_t*>(sorted + TILE2);  // [F2]
   __shared__ uint32_t wave_tmp[32];
   const uint32_t F1 = d.F1, F2 = d.F2, tid = threadIdx.x;
-  const uint32_t tiles_b = uniform_load(d.build.tile_start + F1);
-  const uint32_t total = tiles_b + uniform_load(d.probe.tile_start + F1);
+  const uint32_t tiles_b = (d.sides & 1) ? uniform_load(d.build.tile_start + F1) : 0u;
+  const uint32_t total = tiles_b + ((d.sides & 2) ? uniform_load(d.probe.tile_start + F1) : 0u);
   const uint64_t mask2 = radix_mask(d.bits2, d.bits1);
   uint32_t t = blockIdx.x;
   if (t >= total) return;  // uniform
@@ -187,14 +188,17 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
 
 constexpr uint32_t kMaxRuns = 64;
 
+// Runs of one side of item (p, q): with two passes, one per pass-2 tile of
+// pass-1 partition p; with one pass, partition p cut into tile2-tuple pieces.
 __device__ __forceinline__ uint32_t run_count(const PipelineDevice& d, const PipelineSide& s, uint32_t p) {
-  return d.bits2 ? uniform_load(s.tile_start + p + 1) - uniform_load(s.tile_start + p) : 1u;
+  if (d.bits2) return uniform_load(s.tile_start + p + 1) - uniform_load(s.tile_start + p);
+  return (uniform_load(s.plen + p) + d.tile2 - 1) / d.tile2;
 }
 
 // Loaded, not yet combined, descriptor of run r0 + lane of item (p, q).  With
 // two passes run r is pass-2 digit q's range inside tile r of pass-1
 // partition p: a = first tuple of the tile, b0 = its meta word (start | end
-// << 16).  With one pass the item is partition p, one run: a = its first
+// << 16).  With one pass run r is piece r of partition p: a = its first
 // tuple, [b0, b1) = [0, its length).  Every lane loads from an in-bounds address
 // (lanes past the last run repeat it) and combining is left to the consumer:
 // no branch and no use follows the loads, so a prefetch does not make the
@@ -216,9 +220,10 @@ __device__ __forceinline__ RawRun raw_run(const PipelineDevice& d, const Pipelin
     x.b0 = s.meta[static_cast<uint64_t>(tile) * d.F2 + q];
     x.b1 = 0;
   } else {
-    x.a = s.off1[p];
+    const uint32_t plen = uniform_load(s.plen + p);
+    x.a = uniform_load(s.off1 + p) + static_cast<uint64_t>(r) * d.tile2;
     x.b0 = 0;
-    x.b1 = s.plen[p];
+    x.b1 = min(d.tile2, plen - min(plen, r * d.tile2));
   }
   return x;
 }
@@ -369,14 +374,15 @@ constexpr size_t join_wave_bytes() {
 // table is clear on entry and on return.  bt / pt are register scratch (the
 // caller's idle pipeline buffers).
 template <typename K, uint32_t CAP, uint32_t RCAP, uint32_t BPL, uint32_t SPL>
-__device__ __forceinline__ void join_item_general(const PipelineDevice& d, uint32_t p, uint32_t q, K* table_keys,
-                                                  const WaveRuns& rb, const WaveRuns& rp, const Tuple<K>* rel_b,
-                                                  const Tuple<K>* rel_p, Tuple<K> (&bt)[BPL], Tuple<K> (&pt)[SPL],
-                                                  uint64_t& cnt, uint64_t& sb, uint64_t& sp) {
+__device__ __forceinline__ void join_item_general(const PipelineDevice& d, uint32_t p, uint32_t q, uint32_t pr0,
+                                                  uint32_t pr1, K* table_keys, const WaveRuns& rb,
+                                                  const WaveRuns& rp, const Tuple<K>* rel_b, const Tuple<K>* rel_p,
+                                                  Tuple<K> (&bt)[BPL], Tuple<K> (&pt)[SPL], uint64_t& cnt,
+                                                  uint64_t& sb, uint64_t& sp) {
+  // every build tuple of the item joins the probe runs [pr0, pr1)
   const WaveTable<K, K, CAP> tab{table_keys, table_keys + CAP};
   const uint32_t lane = lane_id();
   const uint32_t nb = run_count(d, d.build, p);
-  const uint32_t np = run_count(d, d.probe, p);
   for (uint32_t rb0 = 0; rb0 < nb; rb0 += kMaxRuns) {
     const uint32_t tb = publish_runs(lane_run(d, d.build, p, q, rb0, nb), rb);
     wave_lds_sync();
@@ -394,8 +400,8 @@ __device__ __forceinline__ void join_item_general(const PipelineDevice& d, uint3
           if (bt[k].key != null_key<K>()) tab.insert(bt[k].key, bt[k].value);
       }
       wave_lds_sync();
-      for (uint32_t rp0 = 0; rp0 < np; rp0 += kMaxRuns) {
-        const uint32_t tp = publish_runs(lane_run(d, d.probe, p, q, rp0, np), rp);
+      for (uint32_t rp0 = pr0; rp0 < pr1; rp0 += kMaxRuns) {
+        const uint32_t tp = publish_runs(lane_run(d, d.probe, p, q, rp0, pr1), rp);
         wave_lds_sync();
         for (uint32_t f0 = 0; f0 < tp; f0 += SPL * 64) {
 #pragma unroll
@@ -515,50 +521,85 @@ struct WaveLds {
   }
 };
 
-// The items the register-resident join deferred (more than 64 runs or more
-// tuples than its registers hold on a side): it sets overflow[item] = 1 and
-// the counter word kOverflowAny.  When that word is zero every block only
-// adds a zero aggregate; otherwise waves take tickets of 256 items, join
-// every flagged one (one wave per item) and clear its flag, so the flags are
-// all zero again for the next join.
+// A deferred item's probe runs are split into work units of about
+// kUnitTuples probe tuples (at most one unit per run), so that a partition
+// that holds a heavy key is joined by many waves; every unit builds the
+// item's whole build side.
+constexpr uint64_t kUnitTuples = 65536;
+
+// Turns the items the register-resident join deferred (more than 64 runs or
+// more tuples than its registers hold on a side: it set overflow[item] = 1
+// and the counter word kOverflowAny) into work units {item, first run, end
+// run}.  Waves take tickets of 64 items and clear the flags they find, so
+// the flags are all zero again for the next join.
+__global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d) {
+  if (uniform_load(d.work_counter + kOverflowAny) == 0) return;
+  const uint32_t lane = lane_id();
+  const uint32_t F2 = d.bits2 ? d.F2 : 1;
+  const uint32_t items = d.F1 * F2;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(d.work_counter + kOverflowTicket, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (static_cast<uint64_t>(t) * 64 >= items) break;
+    const uint32_t i = t * 64 + lane;
+    const bool flagged = i < items && d.overflow[i] != 0;
+    uint64_t m = __builtin_amdgcn_ballot_w64(flagged);
+    if (flagged) d.overflow[i] = 0;
+    while (m) {
+      const uint32_t item = t * 64 + static_cast<uint32_t>(__builtin_ctzll(m));
+      m &= m - 1;
+      const uint32_t p = item / F2, q = item % F2;
+      const uint32_t np = run_count(d, d.probe, p);
+      uint64_t tuples = 0;
+      for (uint32_t r0 = 0; r0 < np; r0 += kMaxRuns) tuples += lane_run(d, d.probe, p, q, r0, np).len;
+      tuples = wave_broadcast_last(wave_inclusive_scan(tuples));
+      uint32_t units = static_cast<uint32_t>(min<uint64_t>((tuples + kUnitTuples - 1) / kUnitTuples, np));
+      units = max(units, 1u);
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(d.work_counter + kUnitCount, units);
+      base = __builtin_amdgcn_readfirstlane(base);
+      for (uint32_t u = lane; u < units; u += 64) {
+        const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(u) * np / units);
+        const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(u + 1) * np / units);
+        d.units[base + u] = make_uint4(item, r0, r1, 0);
+      }
+    }
+  }
+}
+
+// Joins the work units (one wave per unit, tickets of up to 8 units).  When
+// no item was deferred every block only adds a zero aggregate.
 template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL>
 __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice d) {
   using T = Tuple<K>;
   constexpr uint32_t RCAP = CAP * 3 / 4;
-  constexpr uint32_t PER_LANE = 4;  // flags per lane and ticket
   extern __shared__ uint64_t lds_u64[];
   const WaveLds<K, CAP> l(lds_u64);
   const uint32_t lane = lane_id();
   const uint32_t F2 = d.bits2 ? d.F2 : 1;
-  const uint32_t items = d.F1 * F2;
   const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
   const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
   uint64_t cnt = 0, sb = 0, sp = 0;
-  if (uniform_load(d.work_counter + kOverflowAny) != 0) {
-    bool cleared = false;
+  const uint32_t n = uniform_load(d.work_counter + kOverflowAny) ? uniform_load(d.work_counter + kUnitCount) : 0;
+  if (n) {
+    WaveTable<K, K, CAP>{l.table_keys, l.table_keys + CAP}.clear();
+    wave_lds_sync();
+    const uint32_t per_wave = n / (gridDim.x * WAVES * 16);
+    const uint32_t tk = per_wave < 1 ? 1 : (per_wave > 8 ? 8 : per_wave);
     T bt[BPL], pt[SPL];
     for (;;) {
       uint32_t t = 0;
-      if (lane == 0) t = atomicAdd(d.work_counter + kOverflowTicket, 1u);
+      if (lane == 0) t = atomicAdd(d.work_counter + kUnitTicket, tk);
       t = __builtin_amdgcn_readfirstlane(t);
-      if (static_cast<uint64_t>(t) * 64 * PER_LANE >= items) break;
-      for (uint32_t r = 0; r < PER_LANE; ++r) {
-        const uint32_t base = (t * PER_LANE + r) * 64;
-        const uint32_t i = base + lane;
-        const bool flagged = i < items && d.overflow[i] != 0;
-        uint64_t m = __builtin_amdgcn_ballot_w64(flagged);
-        if (flagged) d.overflow[i] = 0;
-        if (m && !cleared) {
-          WaveTable<K, K, CAP>{l.table_keys, l.table_keys + CAP}.clear();
-          wave_lds_sync();
-          cleared = true;
-        }
-        while (m) {
-          const uint32_t item = base + static_cast<uint32_t>(__builtin_ctzll(m));
-          m &= m - 1;
-          join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, l.table_keys, l.rb, l.rp, rel_b, rel_p,
-                                                    bt, pt, cnt, sb, sp);
-        }
+      if (t >= n) break;
+      const uint32_t e = min(t + tk, n);
+      for (uint32_t u = t; u < e; ++u) {
+        const uint4 unit = d.units[u];
+        const uint32_t item = __builtin_amdgcn_readfirstlane(unit.x);
+        join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, __builtin_amdgcn_readfirstlane(unit.y),
+                                                  __builtin_amdgcn_readfirstlane(unit.z), l.table_keys, l.rb, l.rp,
+                                                  rel_b, rel_p, bt, pt, cnt, sb, sp);
       }
     }
   }
@@ -680,8 +721,8 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
         gp = rel_p;
         staged = true;
       } else if (!empty && lane == 0) {
-        // does not fit the registers: deferred to join_general_kernel (an
-        // empty item has no match)
+        // does not fit the registers: deferred to join_units_kernel and
+        // join_general_kernel (an empty item has no match)
         d.overflow[cur] = 1;
         d.work_counter[kOverflowAny] = 1;
       }
@@ -722,14 +763,47 @@ size_t sort2_lds_bytes(uint32_t F2) {
   return TILE2 * sizeof(Tuple<K>) + F2 * 4 + 16;
 }
 
+// dst = the sum of src[0..n) (one block; wrapping 64-bit sums).
+__global__ __launch_bounds__(256) void sum_aggregates_kernel(const JoinAggregate* src, uint32_t n,
+                                                             JoinAggregate* dst) {
+  __shared__ uint64_t red[3][4];
+  uint64_t c = 0, b = 0, p = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    c += src[i].count;
+    b += src[i].sum_build;
+    p += src[i].sum_probe;
+  }
+  c = wave_broadcast_last(wave_inclusive_scan(c));
+  b = wave_broadcast_last(wave_inclusive_scan(b));
+  p = wave_broadcast_last(wave_inclusive_scan(p));
+  const uint32_t w = threadIdx.x / 64;
+  if (lane_id() == 0) {
+    red[0][w] = c;
+    red[1][w] = b;
+    red[2][w] = p;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    dst->count = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    dst->sum_build = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    dst->sum_probe = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- launches ----
 
+hipError_t launch_sum_aggregates(const JoinAggregate* src, uint32_t n, JoinAggregate* dst, hipStream_t stream) {
+  hipLaunchKernelGGL(sum_aggregates_kernel, dim3(1), dim3(256), 0, stream, src, n, dst);
+  return hipGetLastError();
+}
+
 uint32_t pipeline_tile2(KeyType t) { (void)t; return 8192u; }
 
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream) {
-  hipLaunchKernelGGL(plan_kernel, dim3(2), dim3(1024), 2 * d.F1 * 8, stream, d);
+  const uint32_t blocks = d.sides == 3 ? 2u : 1u;
+  hipLaunchKernelGGL(plan_kernel, dim3(blocks), dim3(1024), 2 * d.F1 * 8, stream, d);
   return hipGetLastError();
 }
 
@@ -756,25 +830,37 @@ size_t pipeline_sort2_lds(KeyType t, uint32_t F2) {
   return sort2_lds_bytes<int64_t, 8192>(F2);
 }
 
-template <typename K, uint32_t CAP, uint32_t WAVES>
+template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t GCAP, uint32_t GWAVES>
 hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, hipStream_t stream) {
   constexpr uint32_t PL = 10;  // tuples per lane and side: two buffers fit 2 waves per SIMD
-  const size_t lds = join_lds_bytes<K, CAP, WAVES>();
-  auto kern = deferred ? join_general_kernel<K, CAP, WAVES, 12, 8> : join_kernel<K, CAP, WAVES, PL, PL, true, true>;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+  if (!deferred) {
+    const size_t lds = join_lds_bytes<K, CAP, WAVES>();
+    auto kern = join_kernel<K, CAP, WAVES, PL, PL, true, true>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(256), 0, stream, d);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // deferred items are large: bigger tables, fewer waves (grid <= aggregates)
+  const size_t lds = join_lds_bytes<K, GCAP, GWAVES>();
+  auto gen = join_general_kernel<K, GCAP, GWAVES, 12, 8>;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(gen), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
+  hipLaunchKernelGGL(gen, dim3(grid), dim3(GWAVES * 64), lds, stream, d);
   return hipGetLastError();
 }
 
 hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, bool deferred, uint32_t grid,
                                 hipStream_t stream) {
   if (t == KeyType::I32) {
-    if (large_tables) return launch_join_t<int32_t, 8192, 2>(d, deferred, grid, stream);
-    return launch_join_t<int32_t, 1024, 4>(d, deferred, grid, stream);
+    if (large_tables) return launch_join_t<int32_t, 8192, 2, 8192, 2>(d, deferred, grid, stream);
+    return launch_join_t<int32_t, 1024, 4, 4096, 4>(d, deferred, grid, stream);
   }
-  if (large_tables) return launch_join_t<int64_t, 4096, 2>(d, deferred, grid, stream);
-  return launch_join_t<int64_t, 1024, 4>(d, deferred, grid, stream);
+  if (large_tables) return launch_join_t<int64_t, 4096, 2, 4096, 2>(d, deferred, grid, stream);
+  return launch_join_t<int64_t, 1024, 4, 2048, 4>(d, deferred, grid, stream);
 }
 
 }  // namespace fij

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "internal.hpp"
 
 namespace fij {
@@ -34,13 +36,16 @@ struct PipelineSide {
 
 // Join work counters, each on its own 128-byte line: one per XCD (work
 // groups are dispatched to the 8 XCDs round-robin), then the ticket counter
-// of the kernel that joins the deferred items, then a word that is non-zero
-// when any item was deferred.
+// over the deferred-item flags, a word that is non-zero when any item was
+// deferred, the number of work units made of them and the ticket counter
+// over those units.
 constexpr uint32_t kJoinCounters = 8;
 constexpr uint32_t kJoinCounterStride = 32;
 constexpr uint32_t kOverflowTicket = kJoinCounters * kJoinCounterStride;
 constexpr uint32_t kOverflowAny = (kJoinCounters + 1) * kJoinCounterStride;
-constexpr uint32_t kJoinCounterWords = (kJoinCounters + 2) * kJoinCounterStride;
+constexpr uint32_t kUnitCount = (kJoinCounters + 2) * kJoinCounterStride;
+constexpr uint32_t kUnitTicket = (kJoinCounters + 3) * kJoinCounterStride;
+constexpr uint32_t kJoinCounterWords = (kJoinCounters + 4) * kJoinCounterStride;
 
 struct PipelineDevice {
   PipelineSide build;
@@ -49,17 +54,32 @@ struct PipelineDevice {
   uint32_t bits1, bits2;
   uint32_t pad;
   uint32_t tile2;
+  uint32_t sides;             // bit 0: build, bit 1: probe -- the sides the plan and pass-2 kernels process
   uint32_t* work_counter;     // [kJoinCounterWords]
   uint8_t* overflow;          // [F1 * F2] 1 = item deferred to the general join (zero between joins)
+  uint4* units;               // [unit capacity] {item, first probe run, end probe run, 0}
   JoinAggregate* aggregates;  // [agg_len]
   uint32_t agg_len;
 };
+
+// Allocates one side's buffers (capacity n tuples, F1 x F2 partitions, G1
+// chunks; rel2 and meta only for two passes, meta with `tiles` + 1 rows) and
+// records every allocation in `allocs`.
+hipError_t alloc_pipeline_side(PipelineSide& s, KeyType t, uint64_t n, uint32_t F1, uint32_t F2, uint32_t G1,
+                               bool two_pass, uint32_t tiles, std::vector<void*>& allocs);
+// Pass 1 of one side: contiguous histogram + offsets scan, then the scatter.
+hipError_t enqueue_side_prefix_sum(const PipelineDevice& d, PipelineSide& s, KeyType t, uint32_t block,
+                                   hipStream_t stream);
+hipError_t enqueue_side_scatter(const PipelineDevice& d, PipelineSide& s, KeyType t, uint32_t block,
+                                uint32_t lds_total, hipStream_t stream);
 
 uint32_t pipeline_tile2(KeyType t);
 size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
 // Pass 2 of both relations in one persistent launch of `grid` blocks.
 hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t grid, hipStream_t stream);
+// dst = the sum of the n per-block aggregates src (one block).
+hipError_t launch_sum_aggregates(const JoinAggregate* src, uint32_t n, JoinAggregate* dst, hipStream_t stream);
 // The register-resident join of every work item (deferred = false), then
 // the join of the items it deferred (deferred = true).
 hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, bool deferred, uint32_t grid,

@@ -29,6 +29,7 @@ struct fij_join_plan {
   uint32_t max_tiles[2];
   uint32_t sort2_grid;
   uint32_t join_grid;
+  uint32_t g1_cap;
   uint32_t block1;
   uint32_t lds_total;
   bool large_tables;
@@ -78,32 +79,8 @@ void release(fij_join_plan* p) {
 }
 
 fij_status alloc_side(fij_join_plan* p, PipelineSide& s, uint64_t n, uint32_t tiles) {
-  const uint32_t F1 = p->d.F1, G1 = p->d.G1, F2 = p->d.F2;
-  const uint32_t tb = tuple_bytes(p->kt);
-  s.n = n;
-  s.padded_len1 = n + static_cast<uint64_t>(F1) * p->d.pad;
-  s.scan1_len = scan_state_words(static_cast<uint64_t>(F1) * G1);
-  fij_status st;
-  if ((st = alloc(p, &s.off1, F1))) return st;
-  if ((st = alloc(p, &s.local1, static_cast<uint64_t>(G1) * F1))) return st;
-  if ((st = alloc(p, &s.counts1, static_cast<uint64_t>(F1) * G1))) return st;
-  if ((st = alloc(p, &s.scan1, s.scan1_len))) return st;
-  uint8_t* rel1 = nullptr;
-  if ((st = alloc(p, &rel1, s.padded_len1 * tb))) return st;
-  s.rel1 = rel1;
-  if ((st = alloc(p, &s.plen, F1))) return st;
-  if ((st = alloc(p, &s.pstart, F1 + 1))) return st;
-  if ((st = alloc(p, &s.tile_start, F1 + 1))) return st;
-  if (p->d.bits2) {
-    uint8_t* rel2 = nullptr;
-    if ((st = alloc(p, &rel2, n * tb))) return st;
-    s.rel2 = rel2;
-    if ((st = alloc(p, &s.meta, (static_cast<uint64_t>(tiles) + 1) * F2))) return st;
-  } else {
-    s.rel2 = nullptr;
-    s.meta = nullptr;
-  }
-  return FIJ_OK;
+  const hipError_t e = alloc_pipeline_side(s, p->kt, n, p->d.F1, p->d.F2, p->d.G1, p->d.bits2 != 0, tiles, p->allocs);
+  return e == hipSuccess ? FIJ_OK : hip_fail(e, "hipMalloc");
 }
 
 }  // namespace
@@ -141,8 +118,18 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
   p->d.F1 = 1u << cfg->radix_bits_1;
   p->d.F2 = 1u << cfg->radix_bits_2;
   p->d.G1 = cfg->pass1_chunks ? cfg->pass1_chunks : cus;
+  if (!cfg->pass1_chunks) {
+    // automatic: one chunk per compute unit, fewer for short relations
+    auto fits = [&](uint64_t n, uint32_t chunks) {
+      uint64_t c = 0;
+      return fij_input_chunk_size(static_cast<fij_type>(cfg->type), n, chunks, &c) == FIJ_OK && c * (chunks - 1) <= n;
+    };
+    while (p->d.G1 > 1 && !(fits(cfg->build_len, p->d.G1) && fits(cfg->probe_len, p->d.G1))) --p->d.G1;
+  }
+  p->g1_cap = p->d.G1;
   p->d.pad = pad_tuples(kt);
   p->d.tile2 = pipeline_tile2(kt);
+  p->d.sides = 3;
   p->block1 = cfg->pass1_block ? std::min<uint32_t>(cfg->pass1_block, 1024) : 1024;
   p->large_tables = cfg->radix_bits_2 == 0;
   p->join_grid = cfg->join_blocks ? cfg->join_blocks : cus * 2;
@@ -171,6 +158,9 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
   if (st == FIJ_OK) st = alloc(p, &p->d.work_counter, kJoinCounterWords);
   const uint64_t items = static_cast<uint64_t>(p->d.F1) * (p->d.bits2 ? p->d.F2 : 1);
   if (st == FIJ_OK) st = alloc(p, &p->d.overflow, items);
+  // work units of deferred items: at most one per item plus one per 65536
+  // probe tuples
+  if (st == FIJ_OK) st = alloc(p, &p->d.units, items + cfg->probe_len / 65536 + 1);
   // the deferred-item flags start at zero (complete before any stream uses
   // them); every join leaves them at zero
   if (st == FIJ_OK && (hipMemset(p->d.overflow, 0, items) != hipSuccess || hipDeviceSynchronize() != hipSuccess))
@@ -197,55 +187,90 @@ fij_status fij_join_plan_destroy(fij_join_plan* plan) {
   return FIJ_OK;
 }
 
-fij_status fij_join_plan_enqueue(fij_join_plan* p, const void* build_keys, const void* build_pays,
-                                 const void* probe_keys, const void* probe_pays, void* stream) {
-  if (!p) return plan_fail(FIJ_INVALID_ARGUMENT, "plan is NULL");
-  if ((p->d.build.n && (!build_keys || !build_pays)) || (p->d.probe.n && (!probe_keys || !probe_pays)))
+}  // extern "C"
+
+namespace {
+
+// Chunks of pass 1 for relations of n_b and n_p tuples: the plan's chunk
+// count, fewer when a relation is too short for it (every chunk but the last
+// has the canonical length, the last one the rest).
+uint32_t exec_chunks(const fij_join_plan* p, uint64_t n_b, uint64_t n_p) {
+  uint32_t g = p->g1_cap;
+  auto fits = [&](uint64_t n, uint32_t chunks) {
+    uint64_t c = 0;
+    return fij_input_chunk_size(static_cast<fij_type>(p->cfg.type), n, chunks, &c) == FIJ_OK && c * (chunks - 1) <= n;
+  };
+  while (g > 1 && !(fits(n_b, g) && fits(n_p, g))) g = g > 64 ? g - 32 : g - 1;
+  return g;
+}
+
+// Enqueues one join of the plan's shape over columns of n_b and n_p tuples
+// (at most the plan's lengths).  With events, the phases are timed and the
+// per-block aggregates are copied to the host; otherwise (capturable: no
+// events, no copies) they are summed into *device_result.
+fij_status enqueue_join(fij_join_plan* p, const void* build_keys, const void* build_pays, uint64_t n_b,
+                        const void* probe_keys, const void* probe_pays, uint64_t n_p, bool events,
+                        fij_join_aggregate* device_result, hipStream_t st) {
+  if (n_b > p->cfg.build_len || n_p > p->cfg.probe_len)
+    return plan_fail(FIJ_INVALID_ARGUMENT, "join of %llu x %llu tuples exceeds the plan's %llu x %llu",
+                     static_cast<unsigned long long>(n_b), static_cast<unsigned long long>(n_p),
+                     static_cast<unsigned long long>(p->cfg.build_len),
+                     static_cast<unsigned long long>(p->cfg.probe_len));
+  if ((n_b && (!build_keys || !build_pays)) || (n_p && (!probe_keys || !probe_pays)))
     return plan_fail(FIJ_INVALID_ARGUMENT, "NULL input column");
   const size_t kb = key_bytes(p->kt);
   for (const void* c : {build_keys, build_pays, probe_keys, probe_pays})
     if (c && reinterpret_cast<uintptr_t>(c) % kb)
       return plan_fail(FIJ_INVALID_ARGUMENT, "input columns must be aligned to their element size");
-  hipStream_t st = static_cast<hipStream_t>(stream);
   PipelineDevice& d = p->d;
   d.build.keys = build_keys;
   d.build.pays = build_pays;
   d.probe.keys = probe_keys;
   d.probe.pays = probe_pays;
-  PLAN_HIP(hipEventRecord(p->ev[0], st));
-  for (PipelineSide* s : {&d.build, &d.probe}) {
-    PLAN_HIP(launch_hist_counts(p->kt, s->keys, nullptr, nullptr, nullptr, s->n, d.G1, d.bits1, 0, s->counts1,
-                                s->scan1, s->scan1_len, p->block1, st));
-    PLAN_HIP(launch_offsets_scan(s->counts1, d.F1, d.G1, d.pad, s->local1, s->off1, s->scan1, st));
-  }
-  PLAN_HIP(hipEventRecord(p->ev[1], st));
-  for (PipelineSide* s : {&d.build, &d.probe}) {
-    PartitionLaunch a{};
-    a.type = p->kt;
-    a.keys = s->keys;
-    a.pays = s->pays;
-    a.n = s->n;
-    a.chunks = d.G1;
-    a.radix_bits = d.bits1;
-    a.ignore_bits = 0;
-    a.chunk_offsets = s->local1;
-    a.relation = s->rel1;
-    a.block = p->block1;
-    a.lds_bytes = p->lds_total;
-    a.buffered = true;
-    PLAN_HIP(launch_partition(a, st));
-  }
-  PLAN_HIP(hipEventRecord(p->ev[2], st));
+  d.build.n = n_b;
+  d.probe.n = n_p;
+  d.build.padded_len1 = n_b + static_cast<uint64_t>(d.F1) * d.pad;
+  d.probe.padded_len1 = n_p + static_cast<uint64_t>(d.F1) * d.pad;
+  d.G1 = exec_chunks(p, n_b, n_p);
+  if (events) PLAN_HIP(hipEventRecord(p->ev[0], st));
+  for (PipelineSide* s : {&d.build, &d.probe}) PLAN_HIP(enqueue_side_prefix_sum(d, *s, p->kt, p->block1, st));
+  if (events) PLAN_HIP(hipEventRecord(p->ev[1], st));
+  for (PipelineSide* s : {&d.build, &d.probe})
+    PLAN_HIP(enqueue_side_scatter(d, *s, p->kt, p->block1, p->lds_total, st));
+  if (events) PLAN_HIP(hipEventRecord(p->ev[2], st));
   PLAN_HIP(launch_pipeline_plan(d, st));
   if (d.bits2) PLAN_HIP(launch_pipeline_sort2(d, p->kt, p->sort2_grid, st));
-  PLAN_HIP(hipEventRecord(p->ev[3], st));
+  if (events) PLAN_HIP(hipEventRecord(p->ev[3], st));
   PLAN_HIP(launch_pipeline_join(d, p->kt, p->large_tables, false, p->join_grid, st));
-  PLAN_HIP(hipEventRecord(p->ev[4], st));
+  if (events) PLAN_HIP(hipEventRecord(p->ev[4], st));
   PLAN_HIP(launch_pipeline_join(d, p->kt, p->large_tables, true, p->join_grid, st));
-  PLAN_HIP(hipEventRecord(p->ev[5], st));
-  PLAN_HIP(hipMemcpyAsync(p->host_agg, d.aggregates, sizeof(JoinAggregate) * p->join_grid, hipMemcpyDeviceToHost,
-                          st));
+  if (events) {
+    PLAN_HIP(hipEventRecord(p->ev[5], st));
+    PLAN_HIP(hipMemcpyAsync(p->host_agg, d.aggregates, sizeof(JoinAggregate) * p->join_grid, hipMemcpyDeviceToHost,
+                            st));
+  }
+  if (device_result)
+    PLAN_HIP(launch_sum_aggregates(d.aggregates, p->join_grid, reinterpret_cast<JoinAggregate*>(device_result), st));
   return FIJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fij_status fij_join_plan_enqueue(fij_join_plan* p, const void* build_keys, const void* build_pays,
+                                 const void* probe_keys, const void* probe_pays, void* stream) {
+  if (!p) return plan_fail(FIJ_INVALID_ARGUMENT, "plan is NULL");
+  return enqueue_join(p, build_keys, build_pays, p->cfg.build_len, probe_keys, probe_pays, p->cfg.probe_len, true,
+                      nullptr, static_cast<hipStream_t>(stream));
+}
+
+fij_status fij_join_plan_enqueue_sized(fij_join_plan* p, const void* build_keys, const void* build_pays,
+                                       uint64_t build_len, const void* probe_keys, const void* probe_pays,
+                                       uint64_t probe_len, fij_join_aggregate* device_result, void* stream) {
+  if (!p || !device_result) return plan_fail(FIJ_INVALID_ARGUMENT, "plan or result is NULL");
+  return enqueue_join(p, build_keys, build_pays, build_len, probe_keys, probe_pays, probe_len, false, device_result,
+                      static_cast<hipStream_t>(stream));
 }
 
 fij_status fij_join_plan_result(fij_join_plan* p, void* stream, fij_join_aggregate* result, fij_join_timing* timing) {

@@ -165,6 +165,7 @@ _proto("fij_np_probe_sum", C.c_int, C.c_int, C.c_int, _vp, _u64, _vp, _vp, _u64,
 _proto("fij_gen_permutation", C.c_int, C.c_int, _vp, _u64, _u64, _i64, _vp)
 _proto("fij_gen_uniform", C.c_int, C.c_int, _vp, _u64, _u64, _u64, _i64, _u64, _vp)
 _proto("fij_gen_iota", C.c_int, C.c_int, _vp, _u64, _i64, _vp)
+_proto("fij_gen_zipf", C.c_int, C.c_int, _vp, _u64, _u64, C.c_double, _u64, _u64, _u64, _i64, _vp)
 
 
 def check(status):
@@ -702,6 +703,12 @@ def gen_iota(out, base=0, stream=None):
     check(lib.fij_gen_iota(int(_ktype(out.dtype)), _ptr(out), out.numel(), base, _stream(stream)))
 
 
+def gen_zipf(out, num_ranks, exponent, seed, stream_id, perm_seed, base=1, stream=None):
+    """Zipf(exponent) keys over the keys gen_permutation(num_ranks, perm_seed, base) makes."""
+    check(lib.fij_gen_zipf(int(_ktype(out.dtype)), _ptr(out), out.numel(), num_ranks, float(exponent), seed,
+                           stream_id, perm_seed, base, _stream(stream)))
+
+
 # --------------------------------------------------------------------------
 # radix-join execution method (include/fij.h: fij_join_plan_*)
 
@@ -731,6 +738,7 @@ _proto("fij_join_plan_device_bytes", C.c_int, _vp, C.POINTER(_u64))
 _proto("fij_join_plan_enqueue", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
 _proto("fij_join_plan_result", C.c_int, _vp, _vp, C.POINTER(JoinAggregate), C.POINTER(_Timing))
 _proto("fij_join_plan_execute", C.c_int, _vp, _vp, _vp, _vp, _vp, C.POINTER(JoinAggregate), C.POINTER(_Timing), _vp)
+_proto("fij_join_plan_enqueue_sized", C.c_int, _vp, _vp, _vp, _u64, _vp, _vp, _u64, _vp, _vp)
 
 
 class JoinPlan:
@@ -774,6 +782,161 @@ class JoinPlan:
             "total_ms": t.total_ms, "pass2_ms": t.pass2_ms, "build_probe_ms": t.build_probe_ms,
             "deferred_ms": t.deferred_ms}
 
+    def enqueue_sized(self, bk, bp, sk, sp, device_result, stream=None):
+        """Enqueues a join of bk.numel() x sk.numel() tuples (at most the plan's
+        lengths) whose {count, sums} go to device_result (a device buffer of
+        at least 24 bytes; DeviceArray or raw address)."""
+        addr = device_result if isinstance(device_result, int) else device_result.ptr
+        check(lib.fij_join_plan_enqueue_sized(C.c_void_p(self.h), _ptr(bk), _ptr(bp), bk.numel(), _ptr(sk), _ptr(sp),
+                                              sk.numel(), C.c_void_p(addr), _stream(stream)))
+
     def execute(self, bk, bp, sk, sp, stream=None):
         self.enqueue(bk, bp, sk, sp, stream)
         return self.result(stream)
+
+
+# --------------------------------------------------------------------------
+# batched joins (include/fij.h: fij_join_batch_*)
+
+class JoinDesc(C.Structure):
+    _fields_ = [("build_keys", _vp), ("build_payloads", _vp), ("build_len", _u64), ("probe_keys", _vp),
+                ("probe_payloads", _vp), ("probe_len", _u64)]
+
+
+class JoinBatchConfig(C.Structure):
+    _fields_ = [("type", _u32), ("max_joins", _u32), ("use_graph", _u32), ("reserved", _u32)]
+
+
+class JoinBatchStats(C.Structure):
+    _fields_ = [("gpu_ms", C.c_float), ("prepare_ms", C.c_float), ("total_ms", C.c_float), ("small_joins", _u32),
+                ("partitioned_joins", _u32), ("size_classes", _u32), ("graph", _u32)]
+
+
+_proto("fij_join_autotune", C.c_int, C.c_int, _u64, _u64, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32))
+_proto("fij_join_batch_create", C.c_int, C.POINTER(JoinBatchConfig), C.POINTER(_vp))
+_proto("fij_join_batch_destroy", C.c_int, _vp)
+_proto("fij_join_batch_execute", C.c_int, _vp, _u32, C.POINTER(JoinDesc), _vp, C.POINTER(JoinBatchStats), _vp)
+_proto("fij_join_batch_device_bytes", C.c_int, _vp, C.POINTER(_u64))
+
+
+def autotune(dtype, build_len, probe_len):
+    """(small, radix_bits_1, radix_bits_2) of the size-class rule."""
+    sm, b1, b2 = _u32(), _u32(), _u32()
+    check(lib.fij_join_autotune(int(_ktype(dtype)), build_len, probe_len, C.byref(sm), C.byref(b1), C.byref(b2)))
+    return bool(sm.value), b1.value, b2.value
+
+
+class JoinBatch:
+    """Many independent joins with one submission.  execute() takes a list of
+    (build_keys, build_payloads, probe_keys, probe_payloads) DeviceArrays and
+    returns one (count, sum_build, sum_probe) per join and the stats."""
+
+    def __init__(self, dtype, max_joins, use_graph=True):
+        cfg = JoinBatchConfig(int(_ktype(dtype)), max_joins, 1 if use_graph else 0, 0)
+        h = _vp()
+        check(lib.fij_join_batch_create(C.byref(cfg), C.byref(h)))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.fij_join_batch_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def execute(self, joins, stream=None):
+        n = len(joins)
+        descs = (JoinDesc * max(n, 1))()
+        for i, (rk, rp, sk, sp) in enumerate(joins):
+            descs[i] = JoinDesc(rk.ptr, rp.ptr, rk.numel(), sk.ptr, sp.ptr, sk.numel())
+        res = (JoinAggregate * max(n, 1))()
+        st = JoinBatchStats()
+        check(lib.fij_join_batch_execute(C.c_void_p(self.h), n, descs, C.cast(res, _vp), C.byref(st), _stream(stream)))
+        out = [(res[i].count, res[i].sum_build, res[i].sum_probe) for i in range(n)]
+        stats = {k: getattr(st, k) for k, _ in JoinBatchStats._fields_}
+        return out, stats
+
+    def device_bytes(self):
+        b = _u64()
+        check(lib.fij_join_batch_device_bytes(C.c_void_p(self.h), C.byref(b)))
+        return b.value
+
+
+# --------------------------------------------------------------------------
+# served joins (include/fij.h: fij_join_server_*)
+
+SERVER_MAX_SLOT_CLASSES = 4
+
+
+class JoinServerConfig(C.Structure):
+    _fields_ = [("type", _u32), ("slot_classes", _u32), ("slots", _u32 * SERVER_MAX_SLOT_CLASSES),
+                ("slot_probe_len", _u64 * SERVER_MAX_SLOT_CLASSES), ("max_probe_len", _u64), ("hbm_budget", _u64)]
+
+
+class JoinRequestTimes(C.Structure):
+    _fields_ = [("queued_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+_proto("fij_join_server_create", C.c_int, C.POINTER(JoinServerConfig), C.POINTER(_vp))
+_proto("fij_join_server_destroy", C.c_int, _vp)
+_proto("fij_join_server_add_build", C.c_int, _vp, _vp, _vp, _u64, C.POINTER(_u32))
+_proto("fij_join_server_submit", C.c_int, _vp, _u32, _vp, _vp, _u64, C.POINTER(_u64))
+_proto("fij_join_server_wait", C.c_int, _vp, _u64, C.POINTER(JoinAggregate), C.POINTER(JoinRequestTimes))
+_proto("fij_join_server_memory", C.c_int, _vp, C.POINTER(_u64), C.POINTER(_u32))
+
+
+class JoinServer:
+    """Joins against resident build sides.  slot_classes is a list of
+    (slots, largest probe length); add_build() partitions a build side and
+    keeps it; submit() queues a probe side (DeviceArrays that stay valid until
+    wait() returns) and returns a request id; wait() returns
+    ((count, sum_build, sum_probe), {"queued_ms", "latency_ms"})."""
+
+    def __init__(self, dtype, slot_classes, max_probe_len, hbm_budget=0):
+        if not 0 < len(slot_classes) <= SERVER_MAX_SLOT_CLASSES:
+            raise ValueError("between 1 and %d slot classes" % SERVER_MAX_SLOT_CLASSES)
+        cfg = JoinServerConfig()
+        cfg.type = int(_ktype(dtype))
+        cfg.slot_classes = len(slot_classes)
+        for i, (n, cap) in enumerate(slot_classes):
+            cfg.slots[i] = n
+            cfg.slot_probe_len[i] = cap
+        cfg.max_probe_len = max_probe_len
+        cfg.hbm_budget = hbm_budget
+        h = _vp()
+        check(lib.fij_join_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        self._held = {}
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.fij_join_server_destroy(C.c_void_p(self.h))
+            self.h = None
+            self._held = {}
+
+    def add_build(self, keys, payloads):
+        bid = _u32()
+        check(lib.fij_join_server_add_build(C.c_void_p(self.h), keys.ptr, payloads.ptr, keys.numel(), C.byref(bid)))
+        return bid.value
+
+    def submit(self, build_id, keys, payloads):
+        rid = _u64()
+        check(lib.fij_join_server_submit(C.c_void_p(self.h), build_id, keys.ptr, payloads.ptr, keys.numel(),
+                                         C.byref(rid)))
+        self._held[rid.value] = (keys, payloads)
+        return rid.value
+
+    def wait(self, request_id):
+        res = JoinAggregate()
+        t = JoinRequestTimes()
+        try:
+            check(lib.fij_join_server_wait(C.c_void_p(self.h), request_id, C.byref(res), C.byref(t)))
+        finally:
+            self._held.pop(request_id, None)
+        return (res.count, res.sum_build, res.sum_probe), {"queued_ms": t.queued_ms, "latency_ms": t.latency_ms}
+
+    def memory(self):
+        used, builds = _u64(), _u32()
+        check(lib.fij_join_server_memory(C.c_void_p(self.h), C.byref(used), C.byref(builds)))
+        return used.value, builds.value

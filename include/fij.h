@@ -302,7 +302,7 @@ typedef struct fij_join_plan_config {
   uint32_t type;         /* fij_type */
   uint32_t radix_bits_1; /* pass-1 bits (RadixBits first pass) */
   uint32_t radix_bits_2; /* pass-2 bits; 0 = single pass */
-  uint32_t pass1_chunks; /* pass-1 chunks (= blocks); 0 = one per compute unit */
+  uint32_t pass1_chunks; /* pass-1 chunks (= blocks); 0 = one per compute unit (fewer for short relations) */
   uint32_t pass1_block;  /* threads per pass-1 block; 0 = 1024 */
   uint32_t join_blocks;  /* resident blocks of the join; 0 = two per compute unit */
   uint64_t build_len;    /* |R| */
@@ -332,6 +332,14 @@ fij_status fij_join_plan_device_bytes(const fij_join_plan *plan, uint64_t *bytes
 /* Enqueues one join on `stream`; the result is read with fij_join_plan_result. */
 fij_status fij_join_plan_enqueue(fij_join_plan *plan, const void *build_keys, const void *build_payloads,
                                  const void *probe_keys, const void *probe_payloads, void *stream);
+/* Enqueues one join over columns of build_len and probe_len tuples (at most
+ * the plan's lengths) and writes its {count, sums} to *device_result (device
+ * memory); no events, no host copies and no allocations, so it can be
+ * recorded into a graph.  Joins enqueued on one stream run one after
+ * another; a plan serves one stream at a time. */
+fij_status fij_join_plan_enqueue_sized(fij_join_plan *plan, const void *build_keys, const void *build_payloads,
+                                       uint64_t build_len, const void *probe_keys, const void *probe_payloads,
+                                       uint64_t probe_len, fij_join_aggregate *device_result, void *stream);
 /* Waits for the enqueued join; fills result and (optionally) phase times. */
 fij_status fij_join_plan_result(fij_join_plan *plan, void *stream, fij_join_aggregate *result,
                                 fij_join_timing *timing);
@@ -339,6 +347,100 @@ fij_status fij_join_plan_result(fij_join_plan *plan, void *stream, fij_join_aggr
 fij_status fij_join_plan_execute(fij_join_plan *plan, const void *build_keys, const void *build_payloads,
                                  const void *probe_keys, const void *probe_payloads, fij_join_aggregate *result,
                                  fij_join_timing *timing, void *stream);
+
+/* ------------------------------------------------------ batched joins ---- */
+/* Many independent joins with one submission (no counterpart in the
+ * reference, which joins one pair per run of the radix-join binary).  Joins
+ * whose build side fits one block's LDS table are joined together by one
+ * persistent launch without partitioning; every other join runs the radix
+ * plan of its size class (fij_join_autotune) from a pooled plan per class;
+ * the submission is recorded into one HIP graph when use_graph is set.
+ * Results are per join, identical to joining each pair alone. */
+
+typedef struct fij_join_desc {
+  const void *build_keys;
+  const void *build_payloads;
+  uint64_t build_len;
+  const void *probe_keys;
+  const void *probe_payloads;
+  uint64_t probe_len;
+} fij_join_desc;
+
+typedef struct fij_join_batch_config {
+  uint32_t type;      /* fij_type */
+  uint32_t max_joins; /* joins per execute call */
+  uint32_t use_graph; /* record the submission into a HIP graph */
+  uint32_t reserved;
+} fij_join_batch_config;
+
+typedef struct fij_join_batch_stats {
+  float gpu_ms;      /* device time of the submission */
+  float prepare_ms;  /* host: classification, plan growth, recording */
+  float total_ms;    /* host: whole execute call */
+  uint32_t small_joins;
+  uint32_t partitioned_joins;
+  uint32_t size_classes;
+  uint32_t graph;
+} fij_join_batch_stats;
+
+typedef struct fij_join_batch fij_join_batch;
+
+/* The size-class rule: small = 1 when one block joins the pair from LDS;
+ * otherwise the plan's radix bits for these lengths. */
+fij_status fij_join_autotune(fij_type type, uint64_t build_len, uint64_t probe_len, uint32_t *small,
+                             uint32_t *radix_bits_1, uint32_t *radix_bits_2);
+fij_status fij_join_batch_create(const fij_join_batch_config *config, fij_join_batch **out);
+fij_status fij_join_batch_destroy(fij_join_batch *batch);
+/* Joins joins[0..n) (device columns) and writes results[0..n) (host).  The
+ * first call for a size class larger than any before allocates that class's
+ * plan; later calls reuse it.  Synchronous. */
+fij_status fij_join_batch_execute(fij_join_batch *batch, uint32_t n, const fij_join_desc *joins,
+                                  fij_join_aggregate *results, fij_join_batch_stats *stats, void *stream);
+fij_status fij_join_batch_device_bytes(const fij_join_batch *batch, uint64_t *bytes);
+
+/* ------------------------------------------------------- served joins ---- */
+/* A request loop over resident build sides (no counterpart in the reference,
+ * whose radix-join binary partitions both relations of every join).  Build
+ * sides are partitioned once and kept in device memory; each request brings
+ * a probe side, waits in arrival order for a free slot that fits it (a
+ * preallocated probe workspace with its own stream), partitions its probe
+ * side and joins it with the resident partitions.  Admitted requests overlap
+ * on the device; results equal joining each pair alone.  Memory: slots and
+ * resident builds are charged against hbm_budget (0 = unlimited). */
+
+#define FIJ_SERVER_MAX_SLOT_CLASSES 4
+
+typedef struct fij_join_server_config {
+  uint32_t type;                                         /* fij_type */
+  uint32_t slot_classes;                                 /* <= FIJ_SERVER_MAX_SLOT_CLASSES */
+  uint32_t slots[FIJ_SERVER_MAX_SLOT_CLASSES];           /* slots per class */
+  uint64_t slot_probe_len[FIJ_SERVER_MAX_SLOT_CLASSES];  /* largest probe side of a class's slot */
+  uint64_t max_probe_len;                                /* largest probe side expected (radix bits of builds) */
+  uint64_t hbm_budget;                                   /* bytes; 0 = unlimited */
+} fij_join_server_config;
+
+typedef struct fij_join_request_times {
+  double queued_ms;  /* arrival to admission */
+  double latency_ms; /* arrival to result */
+} fij_join_request_times;
+
+typedef struct fij_join_server fij_join_server;
+
+fij_status fij_join_server_create(const fij_join_server_config *config, fij_join_server **out);
+/* Waits for every admitted request, then releases everything. */
+fij_status fij_join_server_destroy(fij_join_server *server);
+/* Partitions a build side (device columns, read only during the call) and
+ * keeps it resident; synchronous. */
+fij_status fij_join_server_add_build(fij_join_server *server, const void *keys, const void *payloads, uint64_t len,
+                                     uint32_t *build_id);
+/* Queues a request (device probe columns, valid until its result is taken);
+ * returns at once.  Thread-safe. */
+fij_status fij_join_server_submit(fij_join_server *server, uint32_t build_id, const void *probe_keys,
+                                  const void *probe_payloads, uint64_t probe_len, uint64_t *request_id);
+/* Waits for a request's result and its queueing time and latency. */
+fij_status fij_join_server_wait(fij_join_server *server, uint64_t request_id, fij_join_aggregate *result,
+                                fij_join_request_times *times);
+fij_status fij_join_server_memory(const fij_join_server *server, uint64_t *used_bytes, uint32_t *builds);
 
 /* ---------------------------------------------------- data generation ---- */
 /* Seeded replacements for the reference's unseeded generators
@@ -352,6 +454,13 @@ fij_status fij_gen_uniform(fij_type type, void *out, uint64_t n, uint64_t seed, 
                            uint64_t range, void *stream);
 /* out[i] = base + i (payloads). */
 fij_status fij_gen_iota(fij_type type, void *out, uint64_t n, int64_t base, void *stream);
+/* Skewed foreign keys (the datagen Zipf relation, datagen/src/relation.rs:218-290):
+ * out[i] = base + perm(rank_i - 1), rank_i ~ Zipf(exponent) over 1..num_ranks
+ * by rejection-inversion and perm the permutation fij_gen_permutation uses for
+ * num_ranks keys with perm_seed -- so the keys are those of that relation and
+ * the most frequent one sits at a seeded position. */
+fij_status fij_gen_zipf(fij_type type, void *out, uint64_t n, uint64_t num_ranks, double exponent, uint64_t seed,
+                        uint64_t stream_id, uint64_t perm_seed, int64_t base, void *stream);
 
 #ifdef __cplusplus
 }

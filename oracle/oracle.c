@@ -132,6 +132,29 @@ int orc_gen_zipf_ranks(uint64_t *out, uint64_t n, uint64_t num_ranks, double s,
   return 0;
 }
 
+/* Zipf keys over the keys of a permutation relation: rank r (1..N, from
+ * orc_gen_zipf_ranks) becomes base + perm(r - 1), perm the permutation
+ * gen_perm_keys uses for N keys with perm_seed -- so rank 1, the hottest key,
+ * sits at a seeded position among R's keys. */
+int orc_gen_zipf_keys_i32(int32_t *out, uint64_t n, uint64_t num_ranks, double s, uint64_t seed, uint64_t stream,
+                          uint64_t perm_seed, int64_t base) {
+  uint64_t *r = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+  if (!r) return -1;
+  int rc = orc_gen_zipf_ranks(r, n, num_ranks, s, seed, stream);
+  for (uint64_t i = 0; rc == 0 && i < n; ++i) out[i] = (int32_t)(base + (int64_t)orc_feistel_perm(r[i] - 1, num_ranks, perm_seed));
+  free(r);
+  return rc;
+}
+int orc_gen_zipf_keys_i64(int64_t *out, uint64_t n, uint64_t num_ranks, double s, uint64_t seed, uint64_t stream,
+                          uint64_t perm_seed, int64_t base) {
+  uint64_t *r = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+  if (!r) return -1;
+  int rc = orc_gen_zipf_ranks(r, n, num_ranks, s, seed, stream);
+  for (uint64_t i = 0; rc == 0 && i < n; ++i) out[i] = base + (int64_t)orc_feistel_perm(r[i] - 1, num_ranks, perm_seed);
+  free(r);
+  return rc;
+}
+
 /* ======================================================================= */
 /* Partitioning                                                             */
 

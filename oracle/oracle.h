@@ -54,6 +54,11 @@ void orc_gen_iota_i32(int32_t *out, uint64_t n, int64_t base);
 void orc_gen_iota_i64(int64_t *out, uint64_t n, int64_t base);
 int orc_gen_zipf_ranks(uint64_t *out, uint64_t n, uint64_t num_ranks, double exponent,
                        uint64_t seed, uint64_t stream);
+/* base + perm(rank - 1) for Zipf ranks over num_ranks keys (see oracle.c). */
+int orc_gen_zipf_keys_i32(int32_t *out, uint64_t n, uint64_t num_ranks, double s, uint64_t seed, uint64_t stream,
+                          uint64_t perm_seed, int64_t base);
+int orc_gen_zipf_keys_i64(int64_t *out, uint64_t n, uint64_t num_ranks, double s, uint64_t seed, uint64_t stream,
+                          uint64_t perm_seed, int64_t base);
 
 /* ---------------- partitioning ------------------------------------------ */
 uint32_t orc_key_to_partition_i32(int32_t key, uint32_t radix_bits, uint32_t ignore_bits);

@@ -58,6 +58,8 @@ for s in ("i32", "i64"):
     _proto(f"orc_np_probe_lp_{s}", None, _vp, _u64, _vp, _vp, _u64, C.POINTER(Res))
     _proto(f"orc_radix_join_{s}", C.c_int, _vp, _vp, _u64, _vp, _vp, _u64, _u32, _u32, _u32, C.POINTER(Res))
 _proto("orc_gen_zipf_ranks", C.c_int, _vp, _u64, _u64, C.c_double, _u64, _u64)
+for s_ in ("i32", "i64"):
+    _proto(f"orc_gen_zipf_keys_{s_}", C.c_int, _vp, _u64, _u64, C.c_double, _u64, _u64, _u64, _i64)
 _proto("orc_input_chunk_size", C.c_int, _u64, _u32, _u32, C.POINTER(_u64))
 _proto("orc_partition_len", _u64, _vp, _u32, _u32, _u64, _u32, _u32)
 _proto("orc_assign_tasks", None, _vp, _u32, _u32, _u32, _u32, _vp)
@@ -100,6 +102,15 @@ def gen_uniform(n, seed, stream, lo, count, dtype=np.int64):
 def gen_iota(n, base=0, dtype=np.int64):
     out = np.empty(n, dtype)
     getattr(o, f"orc_gen_iota_{suffix(dtype)}")(P(out), n, base)
+    return out
+
+
+def gen_zipf_keys(n, num_ranks, exponent, seed, stream, perm_seed, base=1, dtype=np.int64):
+    """Zipf(exponent) keys over the keys gen_perm(num_ranks, perm_seed, dtype, base) makes."""
+    out = np.empty(n, dtype)
+    rc = getattr(o, f"orc_gen_zipf_keys_{suffix(dtype)}")(P(out), n, num_ranks, exponent, seed, stream, perm_seed,
+                                                          base)
+    assert rc == 0
     return out
 
 

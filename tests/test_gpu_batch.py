@@ -1,0 +1,73 @@
+"""Batched joins (fij_join_batch_*): many independent joins of uneven sizes
+with one submission must give, per join, exactly the result of joining that
+pair alone -- checked against the oracle's sort-merge join and the plain
+plan -- with and without recording the submission into a HIP graph."""
+import numpy as np
+import pytest
+
+import fij
+import oracle_lib as orc
+from gpu_helpers import to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def make_join(i, n_r, factor, dtype, seed=1000):
+    rk = orc.gen_perm(n_r, seed + i, dtype, 1)
+    rp = orc.gen_uniform(n_r, seed + i, 1, -(1 << 20), 1 << 21, dtype)
+    n_s = n_r * factor
+    sk = orc.gen_uniform(n_s, seed + i, 2, 1, n_r + n_r // 4, dtype)  # a fifth of the keys miss
+    sp = orc.gen_iota(n_s, i, dtype)
+    return rk, rp, sk, sp
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_batch_matches_each_join_alone(dtype, use_graph):
+    rng = np.random.default_rng(7)
+    sizes = [int(2 ** rng.uniform(6, 18)) for _ in range(40)] + [0, 1, 5000, 12288, 12289, 1 << 17]
+    host = [make_join(i, max(n, 0), int(rng.integers(4, 17)), dtype) for i, n in enumerate(sizes)]
+    dev = [tuple(to_dev(a) if len(a) else fij.DeviceArray(1, dtype) for a in j) for j in host]
+    # empty columns: pass zero-length views of real buffers
+    dev = [tuple(d if len(h) else d.view(0, 0) for d, h in zip(dj, hj)) for dj, hj in zip(dev, host)]
+    batch = fij.JoinBatch(dtype, len(host), use_graph=use_graph)
+    for _ in range(2):  # the second execution reuses the pooled plans
+        got, stats = batch.execute(dev)
+        for j, (h, g) in enumerate(zip(host, got)):
+            assert g == orc.sort_merge_join(*h), f"join {j} ({len(h[0])} x {len(h[2])})"
+    assert stats["small_joins"] + stats["partitioned_joins"] == len(host)
+    assert stats["small_joins"] > 0 and stats["partitioned_joins"] > 0
+    assert fij.device_status() == 0
+
+
+def test_batch_equals_plain_plan():
+    dtype = np.int64
+    host = [make_join(i, n, 8, dtype) for i, n in enumerate([3000, 40000, 300000])]
+    dev = [tuple(to_dev(a) for a in j) for j in host]
+    got, _ = fij.JoinBatch(dtype, 3).execute(dev)
+    for h, d, g in zip(host, dev, got):
+        small, b1, b2 = fij.autotune(dtype, len(h[0]), len(h[2]))
+        plan = fij.JoinPlan(dtype, len(h[0]), len(h[2]), b1, b2)
+        want, _ = plan.execute(*d)
+        assert g == want
+
+
+def test_batch_null_keys_and_duplicates():
+    dtype = np.int32
+    host = []
+    for i, n in enumerate([2000, 50000]):
+        rk = orc.gen_uniform(n, 5 + i, 1, -3, n // 2, dtype)
+        rp = orc.gen_iota(n, 1, dtype)
+        sk = orc.gen_uniform(4 * n, 5 + i, 2, -3, n // 2, dtype)
+        sp = orc.gen_iota(4 * n, 1, dtype)
+        host.append((rk, rp, sk, sp))
+    got, _ = fij.JoinBatch(dtype, 2).execute([tuple(to_dev(a) for a in j) for j in host])
+    assert got == [orc.sort_merge_join(*h) for h in host]
+
+
+def test_autotune_rule():
+    assert fij.autotune(np.int32, 1000, 16000)[0]
+    small, b1, b2 = fij.autotune(np.int64, 1 << 27, 1 << 27)
+    assert not small and (b1, b2) == (9, 9)
+    small, b1, b2 = fij.autotune(np.int32, 1 << 22, 1 << 26)
+    assert not small and b1 >= 7 and (1 << 22) >> (b1 + b2) <= 640

@@ -352,3 +352,16 @@ def test_zipf_zero_is_uniform():
     h = np.bincount(r.astype(np.int64), minlength=101)[1:]
     chi2 = ((h - 2000) ** 2 / 2000).sum()
     assert chi2 < 180
+
+
+def test_zipf_keys_are_permutation_keys():
+    """Zipf keys map ranks through the permutation relation's keys: the most
+    frequent key is that relation's key at index 0 (rank 1)."""
+    N = 4096
+    keys = orc.gen_zipf_keys(100_000, N, 1.0, seed=2, stream=1, perm_seed=77, base=1, dtype=np.int64)
+    rel = orc.gen_perm(N, 77, np.int64, 1)
+    assert np.isin(keys, rel).all()
+    vals, counts = np.unique(keys, return_counts=True)
+    assert vals[np.argmax(counts)] == rel[0]
+    ranks = orc.gen_zipf_ranks(100_000, N, 1.0, seed=2, stream=1)
+    assert np.array_equal(np.sort(counts)[::-1][:5], np.sort(np.bincount(ranks.astype(np.int64)))[::-1][:5])
