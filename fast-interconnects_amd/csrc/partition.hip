@@ -143,6 +143,11 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
     for (int k = 0; k < V; ++k) {
       act[k] = t0 + static_cast<uint64_t>(k) * BLOCK + tid < len;
       d[k] = key_to_partition<K>(key[k], mask, p.ignore_bits);
+      // The payloads are used on some paths only; using them here on every
+      // path retires their loads in this tile's one wait, so the next
+      // tile's loads (issued into the same registers later) never wait for
+      // the loads in flight behind them.
+      asm volatile("" ::"v"(pay[k]));
     }
     // A: rank
 #pragma unroll
@@ -215,16 +220,20 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   };
 
   if (len) {
+    // Pairs of tiles with no exit between them: every path into the loop
+    // head has consumed the (kb, pb) loads, so issuing the next ones never
+    // waits for the (ka, pa) loads in flight.  Loads past the end are
+    // clamped to the last element.
+    const uint64_t tiles = (len + TILE - 1) / TILE;
     K ka[V], pa[V], kb[V], pb[V];
     load_tile(ka, pa, 0);
-    for (uint64_t t0 = 0;; t0 += 2 * TILE) {
-      load_tile(kb, pb, t0 + TILE);
-      process(ka, pa, t0);
-      if (t0 + TILE >= len) break;
-      load_tile(ka, pa, t0 + 2 * TILE);
-      process(kb, pb, t0 + TILE);
-      if (t0 + 2 * TILE >= len) break;
+    for (uint64_t t = 0; t + 2 <= tiles; t += 2) {
+      load_tile(kb, pb, (t + 1) * TILE);
+      process(ka, pa, t * TILE);
+      load_tile(ka, pa, (t + 2) * TILE);
+      process(kb, pb, (t + 1) * TILE);
     }
+    if (tiles & 1) process(ka, pa, (tiles - 1) * TILE);
   }
   // Write the partially filled last line of every partition.
   for (uint32_t idx = tid; idx < F * L; idx += BLOCK) {
