@@ -61,6 +61,11 @@ def parse():
     return ap.parse_args()
 
 
+def log(msg):
+    """Progress of a long workload (stderr, so the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(log2n, bits1, bits2, threads):
     """The oracle's multi-threaded radix join (C restatement of the reference
     CPU partitioner + per-partition hash join) on a bounded sample."""
@@ -254,6 +259,7 @@ def batched_main(a):
             t_reused += (time.perf_counter() - t0) * step
             checked += 1
         del joins, arena
+        log(f"wave of {len(w)} joins done ({checked} checked against the plain launch)")
     value = total / t_batch
     out = {
         "metric": METRIC, "value": value, "unit": "tuples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -304,6 +310,7 @@ def served_main(a):
         fij.check(fij.lib.fij_memcpy(fij.C.c_void_p(p.ptr), fij.C.c_void_p(k.ptr), k.nbytes(), None))
         ids.append(srv.add_build(k, p))
     t_register = time.perf_counter() - t0
+    log(f"{a.builds} build sides registered in {t_register:.2f} s")
     del scratch_k, scratch_p
     probes = []
     for j in range(a.probe_inputs):
@@ -333,6 +340,7 @@ def served_main(a):
         t_plain.append(time.perf_counter() - t0)
         assert g == alone[j], f"probe input {j}: served {alone[j]} != plain {g}"
     del bk
+    log(f"{a.probe_inputs} probe inputs joined alone and by the plain launch")
     stream = rng.integers(0, a.probe_inputs, a.requests)
     tuples = np.array([int(n_b[probes[j][0]]) + int(n_p[j]) for j in range(a.probe_inputs)], dtype=np.float64)
 
@@ -373,6 +381,7 @@ def served_main(a):
     run(None)  # warm-up
     wall_sat, lat_sat = run(None)
     capacity = len(stream) / wall_sat
+    log(f"saturated: {len(stream)} requests in {wall_sat:.2f} s")
     rate = a.load * capacity
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(stream)))
     wall, lat = run(arrivals)

@@ -220,20 +220,26 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   };
 
   if (len) {
-    // Pairs of tiles with no exit between them: every path into the loop
-    // head has consumed the (kb, pb) loads, so issuing the next ones never
-    // waits for the (ka, pa) loads in flight.  Loads past the end are
-    // clamped to the last element.
+    // Three register buffers, loads issued two tiles ahead, unrolled by
+    // three with no exit inside the loop: every path into the loop head has
+    // consumed buffer c, so issuing its next loads never waits for the
+    // loads in flight in a and b.  Loads past the end are clamped to the
+    // last element.
     const uint64_t tiles = (len + TILE - 1) / TILE;
-    K ka[V], pa[V], kb[V], pb[V];
+    K ka[V], pa[V], kb[V], pb[V], kc[V], pc[V];
     load_tile(ka, pa, 0);
-    for (uint64_t t = 0; t + 2 <= tiles; t += 2) {
-      load_tile(kb, pb, (t + 1) * TILE);
+    load_tile(kb, pb, TILE);
+    uint64_t t = 0;
+    for (; t + 3 <= tiles; t += 3) {
+      load_tile(kc, pc, (t + 2) * TILE);
       process(ka, pa, t * TILE);
-      load_tile(ka, pa, (t + 2) * TILE);
+      load_tile(ka, pa, (t + 3) * TILE);
       process(kb, pb, (t + 1) * TILE);
+      load_tile(kb, pb, (t + 4) * TILE);
+      process(kc, pc, (t + 2) * TILE);
     }
-    if (tiles & 1) process(ka, pa, (tiles - 1) * TILE);
+    if (t < tiles) process(ka, pa, t * TILE);
+    if (t + 1 < tiles) process(kb, pb, (t + 1) * TILE);
   }
   // Write the partially filled last line of every partition.
   for (uint32_t idx = tid; idx < F * L; idx += BLOCK) {

@@ -36,6 +36,8 @@ class Status(enum.IntEnum):
     RUNTIME_ERROR = 4
     UNSUPPORTED = 5
     OUT_OF_MEMORY = 6
+    IO_ERROR = 7
+    PARSE_ERROR = 8
 
 
 class KeyType(enum.IntEnum):
@@ -940,3 +942,45 @@ class JoinServer:
         used, builds = _u64(), _u32()
         check(lib.fij_join_server_memory(C.c_void_p(self.h), C.byref(used), C.byref(builds)))
         return used.value, builds.value
+
+
+# --------------------------------------------------------------------------
+# relation files (include/fij.h: fij_relation_file_*)
+
+_proto("fij_relation_file_len", C.c_int, C.c_char_p, C.c_char, C.POINTER(_u64))
+_proto("fij_relation_file_read", C.c_int, C.c_char_p, C.c_char, C.c_int, _vp, _vp, _u64, C.POINTER(_u64))
+_proto("fij_relation_file_write", C.c_int, C.c_char_p, C.c_char, C.c_int, _vp, _vp, _u64)
+
+
+def _delim(d):
+    return b"\0" if d is None else d.encode()
+
+
+def relation_file_len(path, delimiter=None):
+    n = _u64()
+    check(lib.fij_relation_file_len(os.fsencode(path), _delim(delimiter), C.byref(n)))
+    return n.value
+
+
+def read_relation_file(path, dtype, delimiter=None):
+    """(keys, payloads) host arrays of a relation file, as the reference's
+    JoinDataBuilder::build_with_files reads it (two passes: count, read)."""
+    dtype = np.dtype(dtype)
+    n = relation_file_len(path, delimiter)
+    keys = np.empty(n, dtype)
+    pays = np.empty(n, dtype)
+    got = _u64()
+    check(lib.fij_relation_file_read(os.fsencode(path), _delim(delimiter), int(_ktype(dtype)),
+                                     keys.ctypes.data_as(_vp), pays.ctypes.data_as(_vp), n, C.byref(got)))
+    if got.value != n:
+        raise FijError(Status.IO_ERROR, f"{path} changed while it was read")
+    return keys, pays
+
+
+def write_relation_file(path, keys, payloads, delimiter=None):
+    keys = np.ascontiguousarray(keys)
+    payloads = np.ascontiguousarray(payloads, dtype=keys.dtype)
+    if len(keys) != len(payloads):
+        raise ValueError("keys and payloads differ in length")
+    check(lib.fij_relation_file_write(os.fsencode(path), _delim(delimiter), int(_ktype(keys.dtype)),
+                                      keys.ctypes.data_as(_vp), payloads.ctypes.data_as(_vp), len(keys)))

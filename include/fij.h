@@ -34,7 +34,9 @@ typedef enum fij_status {
   FIJ_DEVICE_ERROR = 3,   /* ErrorKind::CudaError */
   FIJ_RUNTIME_ERROR = 4,
   FIJ_UNSUPPORTED = 5,    /* combinations the reference leaves unimplemented!() */
-  FIJ_OUT_OF_MEMORY = 6
+  FIJ_OUT_OF_MEMORY = 6,
+  FIJ_IO_ERROR = 7,       /* ErrorKind::IoError (data-store/src/error.rs) */
+  FIJ_PARSE_ERROR = 8     /* ErrorKind::CsvError: a malformed relation file */
 } fij_status;
 
 /* Key/payload element type: Tuple<i32, i32> or Tuple<i64, i64>. */
@@ -441,6 +443,24 @@ fij_status fij_join_server_submit(fij_join_server *server, uint32_t build_id, co
 fij_status fij_join_server_wait(fij_join_server *server, uint64_t request_id, fij_join_aggregate *result,
                                 fij_join_request_times *times);
 fij_status fij_join_server_memory(const fij_join_server *server, uint64_t *used_bytes, uint32_t *builds);
+
+/* ----------------------------------------------------- relation files ---- */
+/* The reference's relation files (data-store/src/join_data.rs:173-280 reads,
+ * datagen/src/bin/generate.rs:343-365 writes): a header record, then one
+ * `key<delim>value` record per tuple; the reader uses delimiter ' '
+ * (delim = 0 selects it), skips the header and blank lines, accepts CRLF and
+ * gzip-compressed files, and parses each field as an integer of the key
+ * type (optional sign, decimal digits, in range) -- anything else is
+ * FIJ_PARSE_ERROR with the line number.  Columns are host memory. */
+
+/* Number of records (tuples) in a relation file. */
+fij_status fij_relation_file_len(const char *path, char delim, uint64_t *len);
+/* Reads up to capacity tuples into keys / payloads; *len = tuples read. */
+fij_status fij_relation_file_read(const char *path, char delim, fij_type type, void *keys, void *payloads,
+                                  uint64_t capacity, uint64_t *len);
+/* Writes a relation file ("key<delim>value" header); gzip when the path ends in "gz". */
+fij_status fij_relation_file_write(const char *path, char delim, fij_type type, const void *keys,
+                                   const void *payloads, uint64_t len);
 
 /* ---------------------------------------------------- data generation ---- */
 /* Seeded replacements for the reference's unseeded generators

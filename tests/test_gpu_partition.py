@@ -102,6 +102,17 @@ def test_partition_wide_grid(dtype, bits):
     run_partition(3_000_017, dtype, CHUNKED, fij.RadixPartitionAlgorithm.SSWWCv2, bits, grid=512, block=1024)
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("tiles,extra", [(0, 1), (0, -1), (1, 0), (1, 1), (2, 0), (2, 7), (3, 0), (3, 5),
+                                         (4, -1), (5, 3), (6, 0)])
+def test_partition_tile_counts(dtype, tiles, extra):
+    """The buffered scatter works in tiles of 1024 x 4 (i32) / 1024 x 2 (i64)
+    tuples, three tiles per loop turn: one chunk of every remainder."""
+    tile = 1024 * (4 if dtype == np.int32 else 2)
+    n = tiles * tile + (extra if extra >= 0 else tile + extra)
+    run_partition(n, dtype, CHUNKED, fij.RadixPartitionAlgorithm.SSWWCv2, 9, grid=1, block=1024)
+
+
 @pytest.mark.parametrize("hist", [CHUNKED, CONTIG])
 def test_partition_empty(hist):
     run_partition(0, np.int64, hist, fij.RadixPartitionAlgorithm.SSWWCv2, 4, grid=3, block=256)
