@@ -70,8 +70,23 @@ struct PartitionLaunch {
   uint32_t block;             // requested threads per block
   uint32_t lds_bytes;         // LDS the scatter may use per block
   bool buffered;              // software write-combining vs. direct scatter
+  // Optional spill region: partitions spill_from.. are written to
+  // `spill` instead of `relation`, spill[0] standing for tuple index
+  // spill_base of the relation (so the relation's first spill_base tuples
+  // live in `relation`, the rest in `spill`).  spill == nullptr: none.
+  void* spill = nullptr;
+  uint64_t spill_base = 0;
+  uint32_t spill_from = 0;
 };
 hipError_t launch_partition(const PartitionLaunch& a, hipStream_t stream);
+// Pass 1 on the CPU (host memory in and out): the contiguous layout of the
+// GPU's pass 1; offsets[F] receives the partition starts.
+void cpu_partition_contiguous(KeyType t, const void* keys, const void* pays, uint64_t n, uint32_t bits,
+                              uint32_t ignore, uint32_t threads, void* out, uint64_t* offsets);
+// The GPU prefix sums' offsets on the host: chunked offsets[chunks][F], or
+// contiguous offsets[F] and local[chunks][F].
+void cpu_prefix_sum(KeyType t, const void* keys, uint64_t n, uint32_t chunks, uint64_t chunk_len, uint32_t bits,
+                    uint32_t ignore, bool contiguous, uint32_t threads, uint64_t* offsets, uint64_t* local);
 // LDS the buffered scatter needs for this fanout / tuple size / ring depth.
 uint32_t swwc_lds_bytes(uint32_t fanout, uint32_t tuple_bytes, uint32_t ring_lines);
 

@@ -39,7 +39,19 @@ struct ScatterParams {
   const uint64_t* chunk_offsets;
   void* relation;
   uint32_t ring_lines;
+  uintptr_t spill_addr;  // address tuple index 0 would have in the spill region
+  uint32_t spill_from;   // first partition written to the spill region
 };
+
+// Destination of partition d's tuples: the relation or, from spill_from on,
+// the spill region (both indexed by the relation's tuple index).
+template <bool SPILL, typename T>
+__device__ __forceinline__ T* part_dst(const ScatterParams& p, T* rel, uint64_t cbase, uint32_t d) {
+  if constexpr (SPILL) {
+    if (d >= p.spill_from) return reinterpret_cast<T*>(p.spill_addr) + cbase;
+  }
+  return rel;
+}
 
 template <typename K>
 __device__ __forceinline__ void chunk_range(const ScatterParams& p, uint64_t& begin, uint64_t& len) {
@@ -48,7 +60,7 @@ __device__ __forceinline__ void chunk_range(const ScatterParams& p, uint64_t& be
 }
 
 // Direct scatter (reference NC): per-chunk LDS cursors, wave-aggregated ranks.
-template <typename K>
+template <typename K, bool SPILL>
 __global__ __launch_bounds__(1024) void scatter_direct_kernel(ScatterParams p) {
   extern __shared__ uint32_t cursors[];
   const uint32_t F = 1u << p.radix_bits;
@@ -73,11 +85,11 @@ __global__ __launch_bounds__(1024) void scatter_direct_kernel(ScatterParams p) {
     }
     const uint32_t d = key_to_partition<K>(key, mask, p.ignore_bits);
     const uint32_t r = wave_rank_add(cursors, d, act);
-    if (act) store_tuple<K>(rel + r, key, pay);
+    if (act) store_tuple<K>(part_dst<SPILL>(p, rel, cbase, d) + r, key, pay);
   }
 }
 
-template <typename K, int BLOCK, int V>
+template <typename K, int BLOCK, int V, bool SPILL>
 __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   using T = Tuple<K>;
   constexpr uint32_t L = kAlignBytes / sizeof(T);  // tuples per 128-byte line
@@ -170,7 +182,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
           ring[static_cast<size_t>(d[k]) * RL + (pos & (RL - 1))] = t;
           complete = (pos & (L - 1)) == L - 1;
         } else {
-          store_tuple<K>(rel + base[d[k]] + pos, key[k], pay[k]);
+          store_tuple<K>(part_dst<SPILL>(p, rel, cbase, d[k]) + base[d[k]] + pos, key[k], pay[k]);
         }
       }
       const uint64_t cm = ballot(complete);
@@ -201,7 +213,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
           const uint32_t pos = qline[q] * L + (lane % L);
           if (pos >= qlo[q]) {
             const T t = ring[static_cast<size_t>(pp) * RL + (pos & (RL - 1))];
-            store_tuple<K>(rel + base[pp] + pos, t.key, t.value);
+            store_tuple<K>(part_dst<SPILL>(p, rel, cbase, pp) + base[pp] + pos, t.key, t.value);
           }
         }
       }
@@ -247,7 +259,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
     const uint32_t pos = flushed[i] + (idx % L);
     if (pos < fill[i] && pos >= ring_lo[i]) {
       const T t = ring[static_cast<size_t>(i) * RL + (pos & (RL - 1))];
-      store_tuple<K>(rel + base[i] + pos, t.key, t.value);
+      store_tuple<K>(part_dst<SPILL>(p, rel, cbase, i) + base[i] + pos, t.key, t.value);
     }
   }
 }
@@ -255,10 +267,10 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
 // Launches the buffered scatter with the deepest ring (4, 2 or 1 lines per
 // partition) whose LDS fits next to the kernel's static LDS; returns
 // hipErrorNotSupported without launching when even one line does not fit.
-template <typename K, int BLOCK, int V>
+template <typename K, int BLOCK, int V, bool SPILL>
 hipError_t launch_swwc_t(ScatterParams p, uint32_t fanout, uint32_t chunks, uint32_t lds_total,
                          hipStream_t stream) {
-  auto kern = scatter_swwc_kernel<K, BLOCK, V>;
+  auto kern = scatter_swwc_kernel<K, BLOCK, V, SPILL>;
   const uint32_t budget = dynamic_lds_budget(reinterpret_cast<const void*>(kern), lds_total);
   uint32_t ring = 0;
   for (uint32_t r = 4; r >= 1; r /= 2) {
@@ -288,20 +300,28 @@ hipError_t launch_partition_t(const PartitionLaunch& a, hipStream_t stream) {
   p.chunk_offsets = a.chunk_offsets;
   p.relation = a.relation;
   const uint32_t F = 1u << a.radix_bits;
+  const bool spill = a.spill != nullptr && a.spill_from < F;
+  p.spill_from = spill ? a.spill_from : F;
+  p.spill_addr = reinterpret_cast<uintptr_t>(a.spill) - static_cast<uintptr_t>(a.spill_base * sizeof(Tuple<K>));
   if (a.buffered) {
     constexpr int V = sizeof(K) == 4 ? 4 : 2;
     hipError_t e;
-    if (a.block >= 1024)
-      e = launch_swwc_t<K, 1024, V>(p, F, a.chunks, a.lds_bytes, stream);
+    if (spill)
+      e = launch_swwc_t<K, 1024, V, true>(p, F, a.chunks, a.lds_bytes, stream);
+    else if (a.block >= 1024)
+      e = launch_swwc_t<K, 1024, V, false>(p, F, a.chunks, a.lds_bytes, stream);
     else if (a.block >= 512)
-      e = launch_swwc_t<K, 512, V>(p, F, a.chunks, a.lds_bytes, stream);
+      e = launch_swwc_t<K, 512, V, false>(p, F, a.chunks, a.lds_bytes, stream);
     else
-      e = launch_swwc_t<K, 256, V>(p, F, a.chunks, a.lds_bytes, stream);
+      e = launch_swwc_t<K, 256, V, false>(p, F, a.chunks, a.lds_bytes, stream);
     if (e != hipErrorNotSupported) return e;
   }
   // Direct scatter: per-partition cursors only (4 bytes per partition).
   const uint32_t block = a.block > 1024 ? 1024 : a.block;
-  hipLaunchKernelGGL(scatter_direct_kernel<K>, dim3(a.chunks), dim3(block), F * 4, stream, p);
+  if (spill)
+    hipLaunchKernelGGL((scatter_direct_kernel<K, true>), dim3(a.chunks), dim3(block), F * 4, stream, p);
+  else
+    hipLaunchKernelGGL((scatter_direct_kernel<K, false>), dim3(a.chunks), dim3(block), F * 4, stream, p);
   return hipGetLastError();
 }
 

@@ -984,3 +984,106 @@ def write_relation_file(path, keys, payloads, delimiter=None):
         raise ValueError("keys and payloads differ in length")
     check(lib.fij_relation_file_write(os.fsencode(path), _delim(delimiter), int(_ktype(keys.dtype)),
                                       keys.ctypes.data_as(_vp), payloads.ctypes.data_as(_vp), len(keys)))
+
+
+# --------------------------------------------------------------------------
+# host partitioner, pinned host arrays, out-of-core join
+
+class HostArray:
+    """Pinned host memory (fij_mem_alloc_host) with a numpy view `.a`; the
+    GPU can read it in place."""
+
+    def __init__(self, n, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        p = _vp()
+        check(lib.fij_mem_alloc_host(max(self.n, 1) * self.dtype.itemsize, C.byref(p)))
+        self.ptr = p.value
+        buf = (C.c_char * (max(self.n, 1) * self.dtype.itemsize)).from_address(self.ptr)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=self.n)
+
+    @classmethod
+    def from_numpy(cls, x):
+        h = cls(len(x), np.asarray(x).dtype)
+        h.a[:] = x
+        return h
+
+    def numel(self):
+        return self.n
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.a = None
+            lib.fij_mem_free_host(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+_proto("fij_cpu_prefix_sum", C.c_int, C.c_int, C.c_int, _vp, _u64, _u32, _u32, _u32, C.POINTER(_PartitionOffsets))
+_proto("fij_cpu_radix_partition", C.c_int, C.c_int, _vp, _vp, _u64, _u32, _u32, _u32, _vp, _vp)
+
+
+def cpu_prefix_sum(histogram_algorithm, keys, radix_bits, chunks, ignore_bits=0, threads=0):
+    """Offsets of fij_prefix_sum computed on the host: Chunked -> [chunks * F];
+    Contiguous -> ([F], [chunks * F] local offsets)."""
+    keys = np.ascontiguousarray(keys)
+    F = 1 << radix_bits
+    contiguous = int(histogram_algorithm) == 1
+    offs = np.zeros(F if contiguous else chunks * F, np.uint64)
+    local = np.zeros(chunks * F, np.uint64) if contiguous else None
+    o = _PartitionOffsets(offs.ctypes.data, local.ctypes.data if contiguous else None, 0, chunks, chunks,
+                          radix_bits, int(_ktype(keys.dtype)))
+    check(lib.fij_cpu_prefix_sum(int(histogram_algorithm), int(_ktype(keys.dtype)), keys.ctypes.data_as(_vp),
+                                 len(keys), radix_bits, ignore_bits, threads, C.byref(o)))
+    return (offs, local) if contiguous else offs
+
+
+def cpu_radix_partition(keys, payloads, radix_bits, ignore_bits=0, threads=0):
+    """Contiguous pass 1 on the host: (keys, values) of the padded relation
+    (padding slots hold whatever was there: zeros here) and the partition
+    starts."""
+    keys = np.ascontiguousarray(keys)
+    payloads = np.ascontiguousarray(payloads, dtype=keys.dtype)
+    F = 1 << radix_bits
+    pad = 128 // (2 * keys.dtype.itemsize)
+    rel = np.zeros((len(keys) + F * pad, 2), keys.dtype)
+    offs = np.zeros(F, np.uint64)
+    check(lib.fij_cpu_radix_partition(int(_ktype(keys.dtype)), keys.ctypes.data_as(_vp),
+                                      payloads.ctypes.data_as(_vp), len(keys), radix_bits, ignore_bits, threads,
+                                      rel.ctypes.data_as(_vp), offs.ctypes.data_as(_vp)))
+    return rel[:, 0], rel[:, 1], offs
+
+
+class OocConfig(C.Structure):
+    _fields_ = [("type", _u32), ("radix_bits_1", _u32), ("radix_bits_2", _u32), ("pass1_cpu", _u32),
+                ("cpu_threads", _u32), ("reserved", _u32), ("cache_bytes", _u64), ("staging_tuples", _u64)]
+
+
+class OocTiming(C.Structure):
+    _fields_ = [("prefix_sum_ms", C.c_float), ("partition_ms", C.c_float), ("join_ms", C.c_float),
+                ("total_ms", C.c_float), ("cached_build_tuples", _u64), ("cached_probe_tuples", _u64),
+                ("groups", _u32), ("reserved", _u32)]
+
+
+_proto("fij_ooc_join", C.c_int, C.POINTER(OocConfig), _vp, _vp, _u64, _vp, _vp, _u64, C.POINTER(JoinAggregate),
+       C.POINTER(OocTiming))
+
+
+def _col_ptr(x):
+    if isinstance(x, np.ndarray):
+        if not x.flags.c_contiguous:
+            raise ValueError("columns must be contiguous")
+        return x.ctypes.data, len(x)
+    return x.ptr, x.numel()
+
+
+def ooc_join(dtype, bk, bp, sk, sp, bits1, bits2, pass1_cpu=False, cache_bytes=0, staging_tuples=0,
+             cpu_threads=0):
+    """Out-of-core radix join; columns are numpy arrays (CPU pass 1 only),
+    HostArray or DeviceArray.  Returns ((count, sum_build, sum_probe), timing)."""
+    cfg = OocConfig(int(_ktype(dtype)), bits1, bits2, 1 if pass1_cpu else 0, cpu_threads, 0, cache_bytes,
+                    staging_tuples)
+    (bkp, nb), (bpp, _), (skp, ns), (spp, _) = (_col_ptr(x) for x in (bk, bp, sk, sp))
+    res = JoinAggregate()
+    t = OocTiming()
+    check(lib.fij_ooc_join(C.byref(cfg), bkp, bpp, nb, skp, spp, ns, C.byref(res), C.byref(t)))
+    return (res.count, res.sum_build, res.sum_probe), {k: getattr(t, k) for k, _ in OocTiming._fields_}

@@ -444,6 +444,66 @@ fij_status fij_join_server_wait(fij_join_server *server, uint64_t request_id, fi
                                 fij_join_request_times *times);
 fij_status fij_join_server_memory(const fij_join_server *server, uint64_t *used_bytes, uint32_t *builds);
 
+/* ------------------------------------------------ host (CPU) partitioner ---- */
+/* The GPU prefix sum's offsets computed on host threads, from host keys into
+ * host offset buffers (CpuRadixPartitioner::prefix_sum,
+ * sql-ops/src/partition/cpu_radix_partition.rs:46-62 "CPU and GPU histograms
+ * ... can be mixed and matched"; radix_partition.cpp:228-267).  Chunked and
+ * Contiguous give exactly fij_prefix_sum's offsets for max_chunks chunks, so
+ * a caller can copy them into device offsets and run fij_partition, reading
+ * the keys over the interconnect once instead of twice. */
+fij_status fij_cpu_prefix_sum(fij_histogram_algorithm algorithm, fij_type type, const void *keys, uint64_t len,
+                              uint32_t radix_bits, uint32_t ignore_bits, uint32_t threads,
+                              fij_partition_offsets *offsets);
+/* Pass 1 on host threads into host memory, in the contiguous layout
+ * (relation: len + 2^radix_bits * pad tuples; offsets[2^radix_bits] the
+ * partition starts), as the GPU's contiguous pass 1 lays it out
+ * (cpu_radix_partition.rs:364-640, radix_partition.cpp:448-557 write-combining
+ * scatter).  Order within a partition: by thread, then input order. */
+fij_status fij_cpu_radix_partition(fij_type type, const void *keys, const void *payloads, uint64_t len,
+                                   uint32_t radix_bits, uint32_t ignore_bits, uint32_t threads, void *relation,
+                                   uint64_t *offsets);
+
+/* ---------------------------------------------------- out-of-core join ---- */
+/* Relations larger than device memory (radix-join/src/execution_methods/
+ * gpu_triton_join.rs:99-638, the Triton join; with pass1_cpu = 1
+ * cpu_partitioned_radix_join.rs:118-769).  Pass 1 runs on the GPU, reading
+ * the relations in place (device or pinned host memory), or on host threads
+ * (any host memory); its partitions are cached in device memory up to
+ * cache_bytes (split between the relations in proportion to their lengths;
+ * GPU pass 1 only) and spilled to pinned host memory beyond it.  Pass 2 and
+ * the join then run over groups of consecutive pass-1 partitions (at most
+ * staging_tuples tuples of either relation, or one partition when a single
+ * one is longer) on two streams, spilled groups copied into device memory
+ * first.  The result equals the in-memory join's.  Synchronous; allocates
+ * and frees its buffers (not part of the timing). */
+typedef struct fij_ooc_config {
+  uint32_t type;           /* fij_type */
+  uint32_t radix_bits_1;   /* 1..12 */
+  uint32_t radix_bits_2;   /* 1..12 */
+  uint32_t pass1_cpu;      /* 0: pass 1 on the GPU, 1: on host threads */
+  uint32_t cpu_threads;    /* host threads of a CPU pass 1 (0 = all) */
+  uint32_t reserved;
+  uint64_t cache_bytes;    /* device memory for pass-1 partitions */
+  uint64_t staging_tuples; /* group size (0 = 2^24) */
+} fij_ooc_config;
+
+/* RadixJoinPoint (radix-join/src/measurement/data_point.rs:29-74) */
+typedef struct fij_ooc_timing {
+  float prefix_sum_ms;
+  float partition_ms;
+  float join_ms;       /* pass 2 + join over all groups, copies included */
+  float total_ms;
+  uint64_t cached_build_tuples;
+  uint64_t cached_probe_tuples;
+  uint32_t groups;
+  uint32_t reserved;
+} fij_ooc_timing;
+
+fij_status fij_ooc_join(const fij_ooc_config *config, const void *build_keys, const void *build_payloads,
+                        uint64_t build_len, const void *probe_keys, const void *probe_payloads, uint64_t probe_len,
+                        fij_join_aggregate *result, fij_ooc_timing *timing);
+
 /* ----------------------------------------------------- relation files ---- */
 /* The reference's relation files (data-store/src/join_data.rs:173-280 reads,
  * datagen/src/bin/generate.rs:343-365 writes): a header record, then one

@@ -1,0 +1,133 @@
+"""Out-of-core join (fij_ooc_join) and histogram mixing (fij_cpu_prefix_sum
+feeding fij_partition).  The out-of-core join must give the in-memory
+join's result whatever part of pass 1 is cached in device memory, however
+the partitions are grouped, and whether pass 1 runs on the GPU (relations in
+pinned host or device memory) or on host threads."""
+import numpy as np
+import pytest
+
+import fij
+import oracle_lib as orc
+from gpu_helpers import sync, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def make(nr, ns, dtype, seed, dups=False):
+    if dups:  # duplicate and null build keys
+        rk = orc.gen_uniform(nr, seed, 1, -2, nr // 2, dtype)
+    else:
+        rk = orc.gen_perm(nr, seed, dtype, 1)
+    rp = orc.gen_uniform(nr, seed, 2, -(1 << 20), 1 << 21, dtype)
+    sk = orc.gen_uniform(ns, seed, 3, 1, nr + nr // 4, dtype)  # a fifth of the probe keys miss
+    sp = orc.gen_iota(ns, 0, dtype)
+    return rk, rp, sk, sp
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("cache_frac", [0.0, 0.4, 2.0])
+@pytest.mark.parametrize("staging", [0, 1 << 16])
+def test_gpu_pass1_from_pinned_memory(dtype, cache_frac, staging):
+    h = make(1 << 18, 1 << 20, dtype, 5)
+    want = orc.sort_merge_join(*h)
+    cols = [fij.HostArray.from_numpy(a) for a in h]
+    tb = 2 * np.dtype(dtype).itemsize
+    cache = int(cache_frac * (len(h[0]) + len(h[2])) * tb)
+    got, t = fij.ooc_join(dtype, *cols, 7, 5, cache_bytes=cache, staging_tuples=staging)
+    assert got == want
+    if cache_frac == 0.0:
+        assert t["cached_build_tuples"] == 0 and t["cached_probe_tuples"] == 0
+    if cache_frac == 2.0:  # everything fits: padded lengths cached
+        pad = 128 // tb
+        assert t["cached_build_tuples"] == len(h[0]) + 128 * pad
+        assert t["cached_probe_tuples"] == len(h[2]) + 128 * pad
+    if cache_frac == 0.4:
+        assert 0 < t["cached_probe_tuples"] < len(h[2])
+    if staging:
+        assert t["groups"] > 8
+    assert fij.device_status() == 0
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("bits", [(9, 6), (4, 3)])
+def test_cpu_pass1(dtype, bits):
+    h = make(200_000, 900_001, dtype, 8, dups=True)
+    want = orc.sort_merge_join(*h)
+    got, t = fij.ooc_join(dtype, *h, *bits, pass1_cpu=True, cpu_threads=4, staging_tuples=1 << 17)
+    assert got == want
+    assert t["cached_build_tuples"] == 0
+
+
+def test_device_inputs_and_plain_plan():
+    dtype = np.int64
+    h = make(1 << 20, 1 << 22, dtype, 11)
+    d = [to_dev(a) for a in h]
+    got, _ = fij.ooc_join(dtype, *d, 9, 5, cache_bytes=24 << 20, staging_tuples=1 << 19)
+    want, _ = fij.JoinPlan(dtype, 1 << 20, 1 << 22, 9, 5).execute(*d)
+    assert got == want
+    sync()
+
+
+def test_pageable_memory_needs_cpu_pass1():
+    h = make(1 << 12, 1 << 14, np.int32, 3)
+    with pytest.raises(fij.FijError) as e:
+        fij.ooc_join(np.int32, *h, 5, 3)  # numpy arrays: pageable host memory
+    assert e.value.status == fij.Status.INVALID_ARGUMENT
+    got, _ = fij.ooc_join(np.int32, *h, 5, 3, pass1_cpu=True)
+    assert got == orc.sort_merge_join(*h)
+
+
+def test_empty_relations():
+    h = make(1 << 12, 1 << 14, np.int64, 3)
+    e = np.zeros(0, np.int64)
+    assert fij.ooc_join(np.int64, e, e, h[2], h[3], 5, 3, pass1_cpu=True)[0] == (0, 0, 0)
+    assert fij.ooc_join(np.int64, h[0], h[1], e, e, 5, 3, pass1_cpu=True)[0] == (0, 0, 0)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("hist", [0, 1])
+def test_cpu_histogram_with_gpu_partitioner(dtype, hist):
+    """Offsets from the host prefix sum, copied into device offsets, drive
+    the GPU scatter to the same relation layout as the GPU prefix sum."""
+    n, bits, grid = 1_000_003, 9, 40
+    keys = orc.gen_uniform(n, 2, 3, 0, 1 << 30, dtype)
+    pays = orc.gen_iota(n, 0, dtype)
+    kt = fij.KeyType.I32 if dtype == np.int32 else fij.KeyType.I64
+    dk, dp = to_dev(keys), to_dev(pays)
+    algo = fij.RadixPartitionAlgorithm.SSWWCv2
+    rels = []
+    for on_cpu in (False, True):
+        part = fij.GpuRadixPartitioner(hist, algo, fij.RadixBits(bits), grid, 1024)
+        offs = fij.PartitionOffsets(hist, grid, bits, kt)
+        rel = fij.PartitionedRelation(n, hist, bits, grid, kt, fill_null=True)
+        if on_cpu:
+            host = fij.cpu_prefix_sum(hist, keys, bits, grid)
+            if hist == 1:
+                offs.offsets.copy_from_host(host[0])
+                offs.local_offsets.copy_from_host(host[1])
+            else:
+                offs.offsets.copy_from_host(host)
+            offs.s.data_len = n
+        else:
+            part.prefix_sum(fij.RadixPass.First, dk, offs)
+        part.partition(fij.RadixPass.First, dk, dp, offs, rel)
+        sync()
+        rels.append((rel.host_offsets(), *rel.host_relation()))
+    (o0, k0, v0), (o1, k1, v1) = rels
+    np.testing.assert_array_equal(o0, o1)
+    # same partitions, same tuples (order within a partition is not defined)
+    ranges_chunks = grid if hist == 0 else 1
+    pad = orc.pad_len(dtype)
+    F = 1 << bits
+    starts = o0.reshape(ranges_chunks, F) if hist == 0 else o0.reshape(1, F)
+    ends = []
+    for c in range(ranges_chunks):
+        for p in range(F):
+            b = int(starts[c, p])
+            nxt = int(starts[c, p + 1]) - pad if p + 1 < F else (
+                int(starts[c + 1, 0]) - pad if c + 1 < ranges_chunks else len(k0))
+            ends.append((b, nxt))
+    for b, e in ends:
+        a = np.sort(k0[b:e].astype(np.int64) * 2_000_003 + v0[b:e])
+        c = np.sort(k1[b:e].astype(np.int64) * 2_000_003 + v1[b:e])
+        np.testing.assert_array_equal(a, c)
