@@ -530,9 +530,12 @@ constexpr uint64_t kUnitTuples = 65536;
 // Turns the items the register-resident join deferred (more than 64 runs or
 // more tuples than its registers hold on a side: it set overflow[item] = 1
 // and the counter word kOverflowAny) into work units {item, first run, end
-// run}.  Waves take tickets of 64 items and clear the flags they find, so
-// the flags are all zero again for the next join.
-__global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d) {
+// run, path}: an item whose build side fits one fill of a block-shared table
+// (block_rcap tuples) is joined by join_block_kernel (path 0), a larger one
+// by join_general_kernel's wave tables with refills (path 1).  Waves take
+// tickets of 64 items and clear the flags they find, so the flags are all
+// zero again for the next join.
+__global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d, uint32_t block_rcap) {
   if (uniform_load(d.work_counter + kOverflowAny) == 0) return;
   const uint32_t lane = lane_id();
   const uint32_t F2 = d.bits2 ? d.F2 : 1;
@@ -551,9 +554,13 @@ __global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d) {
       m &= m - 1;
       const uint32_t p = item / F2, q = item % F2;
       const uint32_t np = run_count(d, d.probe, p);
-      uint64_t tuples = 0;
+      const uint32_t nb = run_count(d, d.build, p);
+      uint64_t tuples = 0, btuples = 0;
       for (uint32_t r0 = 0; r0 < np; r0 += kMaxRuns) tuples += lane_run(d, d.probe, p, q, r0, np).len;
+      for (uint32_t r0 = 0; r0 < nb; r0 += kMaxRuns) btuples += lane_run(d, d.build, p, q, r0, nb).len;
       tuples = wave_broadcast_last(wave_inclusive_scan(tuples));
+      btuples = wave_broadcast_last(wave_inclusive_scan(btuples));
+      const uint32_t path = btuples <= block_rcap ? 0u : 1u;
       uint32_t units = static_cast<uint32_t>(min<uint64_t>((tuples + kUnitTuples - 1) / kUnitTuples, np));
       units = max(units, 1u);
       uint32_t base = 0;
@@ -562,7 +569,7 @@ __global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d) {
       for (uint32_t u = lane; u < units; u += 64) {
         const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(u) * np / units);
         const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(u + 1) * np / units);
-        d.units[base + u] = make_uint4(item, r0, r1, 0);
+        d.units[base + u] = make_uint4(item, r0, r1, path);
       }
     }
   }
@@ -596,11 +603,89 @@ __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice
       const uint32_t e = min(t + tk, n);
       for (uint32_t u = t; u < e; ++u) {
         const uint4 unit = d.units[u];
+        if (__builtin_amdgcn_readfirstlane(unit.w) == 0) continue;  // joined by join_block_kernel
         const uint32_t item = __builtin_amdgcn_readfirstlane(unit.x);
         join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, __builtin_amdgcn_readfirstlane(unit.y),
                                                   __builtin_amdgcn_readfirstlane(unit.z), l.table_keys, l.rb, l.rp,
                                                   rel_b, rel_p, bt, pt, cnt, sb, sp);
       }
+    }
+  }
+  join_block_aggregate<WAVES>(d, cnt, sb, sp);
+}
+
+// Per-wave LDS of join_block_kernel: one side's run batch.
+constexpr size_t kBlockWaveRunBytes = (kMaxRuns * 8 + (kMaxRuns + 1) * 4 + 15) & ~size_t(15);
+
+// Joins the path-0 work units, one block per unit: the item's whole build
+// side (at most CAP * 3 / 4 tuples) goes into a block-shared LDS table, then
+// the unit's probe runs are probed against it, each wave taking batches of
+// 64 runs and streaming their tuples SPL per lane at a time.  A block of
+// WAVES waves per unit keeps enough probe loads in flight where one wave per
+// unit (join_general_kernel) waits on each batch.
+template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t SPL>
+__global__ __launch_bounds__(WAVES * 64) void join_block_kernel(PipelineDevice d) {
+  using T = Tuple<K>;
+  extern __shared__ uint64_t lds_u64[];
+  __shared__ uint32_t s_unit;
+  K* const tkeys = reinterpret_cast<K*>(lds_u64);
+  const WaveTable<K, K, CAP> tab{tkeys, tkeys + CAP};
+  uint8_t* const runs = reinterpret_cast<uint8_t*>(tkeys + 2 * CAP) + (threadIdx.x / 64) * kBlockWaveRunBytes;
+  const WaveRuns wr{reinterpret_cast<uint64_t*>(runs), reinterpret_cast<uint32_t*>(runs + kMaxRuns * 8)};
+  const uint32_t w = threadIdx.x / 64, lane = lane_id();
+  const uint32_t F2 = d.bits2 ? d.F2 : 1;
+  const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
+  const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
+  uint64_t cnt = 0, sb = 0, sp = 0;
+  const uint32_t n = uniform_load(d.work_counter + kOverflowAny) ? uniform_load(d.work_counter + kUnitCount) : 0;
+  auto clear = [&]() {
+    uint4* k4 = reinterpret_cast<uint4*>(tkeys);
+    for (uint32_t i = threadIdx.x; i < CAP * sizeof(K) / 16; i += WAVES * 64) k4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  };
+  if (n) {
+    clear();
+    for (;;) {
+      if (threadIdx.x == 0) s_unit = atomicAdd(d.work_counter + kBlockTicket, 1u);
+      __syncthreads();  // also orders the clear before the inserts
+      const uint32_t u = s_unit;
+      __syncthreads();  // every wave has read s_unit
+      if (u >= n) break;
+      const uint4 unit = d.units[u];
+      if (__builtin_amdgcn_readfirstlane(unit.w) != 0) continue;  // wave-table unit
+      const uint32_t item = __builtin_amdgcn_readfirstlane(unit.x);
+      const uint32_t p = item / F2, q = item % F2;
+      const uint32_t pr0 = __builtin_amdgcn_readfirstlane(unit.y), pr1 = __builtin_amdgcn_readfirstlane(unit.z);
+      const uint32_t nb = run_count(d, d.build, p);
+      for (uint32_t r0 = w * kMaxRuns; r0 < nb; r0 += WAVES * kMaxRuns) {
+        const uint32_t tb = publish_runs(lane_run(d, d.build, p, q, r0, nb), wr);
+        wave_lds_sync();
+        for (uint32_t e0 = 0; e0 < tb; e0 += 4 * 64) {
+          T t[4];
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) t[k] = load_tuple<K>(rel_b + batch_element(wr, min(e0 + k * 64 + lane, tb - 1)));
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k)
+            if (e0 + k * 64 + lane < tb && t[k].key != null_key<K>()) tab.insert(t[k].key, t[k].value);
+        }
+        wave_lds_sync();
+      }
+      __syncthreads();
+      for (uint32_t r0 = pr0 + w * kMaxRuns; r0 < pr1; r0 += WAVES * kMaxRuns) {
+        const uint32_t tp = publish_runs(lane_run(d, d.probe, p, q, r0, pr1), wr);
+        wave_lds_sync();
+        for (uint32_t e0 = 0; e0 < tp; e0 += SPL * 64) {
+          T t[SPL];
+#pragma unroll
+          for (uint32_t k = 0; k < SPL; ++k)
+            t[k] = load_tuple<K>(rel_p + batch_element(wr, min(e0 + k * 64 + lane, tp - 1)));
+#pragma unroll
+          for (uint32_t k = 0; k < SPL; ++k)
+            if (e0 + k * 64 + lane < tp && t[k].key != null_key<K>()) tab.probe(t[k].key, t[k].value, cnt, sb, sp);
+        }
+        wave_lds_sync();
+      }
+      __syncthreads();
+      clear();
     }
   }
   join_block_aggregate<WAVES>(d, cnt, sb, sp);
@@ -841,9 +926,21 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(256), 0, stream, d);
+  // block-shared tables: 64 KB, two blocks of 8 waves per CU
+  constexpr uint32_t BCAP = 65536 / (2 * sizeof(K));
+  constexpr uint32_t BWAVES = 8;
+  hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(256), 0, stream, d, BCAP * 3 / 4);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  {
+    auto blk = join_block_kernel<K, BCAP, BWAVES, 8>;
+    const size_t lds = BCAP * 2 * sizeof(K) + BWAVES * kBlockWaveRunBytes;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(blk), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(lds));
+    hipLaunchKernelGGL(blk, dim3(grid), dim3(BWAVES * 64), lds, stream, d);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   // deferred items are large: bigger tables, fewer waves (grid <= aggregates)
   const size_t lds = join_lds_bytes<K, GCAP, GWAVES>();
   auto gen = join_general_kernel<K, GCAP, GWAVES, 12, 8>;

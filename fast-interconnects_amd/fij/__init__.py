@@ -761,6 +761,7 @@ class JoinPlan:
         h = _vp()
         check(lib.fij_join_plan_create(C.byref(cfg), C.byref(h)))
         self.h = h
+        self.build_len, self.probe_len = build_len, probe_len
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -773,6 +774,11 @@ class JoinPlan:
         return out.value
 
     def enqueue(self, bk, bp, sk, sp, stream=None):
+        """Joins columns of exactly the plan's lengths (the C call reads that
+        many tuples from each pointer)."""
+        for c, n in ((bk, self.build_len), (bp, self.build_len), (sk, self.probe_len), (sp, self.probe_len)):
+            if c.numel() != n:
+                raise ValueError(f"column of {c.numel()} tuples for a plan of {n}; use enqueue_sized")
         check(lib.fij_join_plan_enqueue(self.h, _ptr(bk), _ptr(bp), _ptr(sk), _ptr(sp), _stream(stream)))
 
     def result(self, stream=None):
@@ -789,7 +795,9 @@ class JoinPlan:
         lengths) whose {count, sums} go to device_result (a device buffer of
         at least 24 bytes; DeviceArray or raw address)."""
         addr = device_result if isinstance(device_result, int) else device_result.ptr
-        check(lib.fij_join_plan_enqueue_sized(C.c_void_p(self.h), _ptr(bk), _ptr(bp), bk.numel(), _ptr(sk), _ptr(sp),
+        if bp.numel() < bk.numel() or sp.numel() < sk.numel():
+            raise ValueError("payload columns shorter than their key columns")
+        check(lib.fij_join_plan_enqueue_sized(self.h, _ptr(bk), _ptr(bp), bk.numel(), _ptr(sk), _ptr(sp),
                                               sk.numel(), C.c_void_p(addr), _stream(stream)))
 
     def execute(self, bk, bp, sk, sp, stream=None):

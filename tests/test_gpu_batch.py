@@ -71,3 +71,22 @@ def test_autotune_rule():
     assert not small and (b1, b2) == (9, 9)
     small, b1, b2 = fij.autotune(np.int32, 1 << 22, 1 << 26)
     assert not small and b1 >= 7 and (1 << 22) >> (b1 + b2) <= 640
+
+
+def test_plan_enqueue_sized_matches_execute():
+    """fij_join_plan_enqueue_sized: joins shorter than the plan, result summed
+    into a device address (the form the batch records into its graph)."""
+    dtype = np.int64
+    plan = fij.JoinPlan(dtype, 1 << 16, 1 << 18, 6, 5)
+    res = fij.DeviceArray(3, np.uint64)
+    for i, (n_r, f) in enumerate([(1 << 16, 4), (1000, 5), (30000, 8)]):
+        h = make_join(i, n_r, f, dtype)
+        d = [to_dev(a) for a in h]
+        plan.enqueue_sized(*d, res)
+        got = tuple(int(x) for x in res.to_host())
+        assert got == orc.sort_merge_join(*h)
+        # a plan of exactly these lengths (execute reads the plan's lengths)
+        want, _ = fij.JoinPlan(dtype, len(h[0]), len(h[2]), 6, 5).execute(*d)
+        assert got == want
+    with pytest.raises(ValueError):  # shorter columns than the plan's: refused before any launch
+        plan.execute(*d)
