@@ -208,11 +208,10 @@ struct RawRun {
   uint32_t b0, b1;
 };
 
-__device__ __forceinline__ RawRun raw_run(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
-                                          uint32_t r0, uint32_t nruns) {
+// Descriptor of run r (in range) of item (p, q).
+__device__ __forceinline__ RawRun raw_run_at(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
+                                             uint32_t r) {
   RawRun x;
-  const uint32_t last = nruns ? nruns - 1 : 0;
-  const uint32_t r = min(r0 + lane_id(), last);
   if (d.bits2) {
     // nruns == 0 reads row tile_start[p] <= tiles: the meta has a spare row
     const uint32_t tile = uniform_load(s.tile_start + p) + r;
@@ -228,20 +227,29 @@ __device__ __forceinline__ RawRun raw_run(const PipelineDevice& d, const Pipelin
   return x;
 }
 
+__device__ __forceinline__ RawRun raw_run(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
+                                          uint32_t r0, uint32_t nruns) {
+  const uint32_t last = nruns ? nruns - 1 : 0;
+  return raw_run_at(d, s, p, q, min(r0 + lane_id(), last));
+}
+
 // Lane-held descriptor of run r0 + lane (length 0 past the last run).
 struct LaneRun {
   uint64_t start;
   uint32_t len;
 };
 
-__device__ __forceinline__ LaneRun finish_run(const PipelineDevice& d, const RawRun& x, uint32_t r0,
-                                              uint32_t nruns) {
-  const bool live = r0 + lane_id() < nruns;
+__device__ __forceinline__ LaneRun finish_run_at(const PipelineDevice& d, const RawRun& x, bool live) {
   if (d.bits2) {
     const uint32_t b = x.b0 & 0xffffu, e = x.b0 >> 16;
     return LaneRun{x.a + b, live ? e - b : 0u};
   }
   return LaneRun{x.a, live ? x.b1 - x.b0 : 0u};
+}
+
+__device__ __forceinline__ LaneRun finish_run(const PipelineDevice& d, const RawRun& x, uint32_t r0,
+                                              uint32_t nruns) {
+  return finish_run_at(d, x, r0 + lane_id() < nruns);
 }
 
 __device__ __forceinline__ LaneRun lane_run(const PipelineDevice& d, const PipelineSide& s, uint32_t p, uint32_t q,
@@ -614,25 +622,66 @@ __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice
   join_block_aggregate<WAVES>(d, cnt, sb, sp);
 }
 
-// Per-wave LDS of join_block_kernel: one side's run batch.
-constexpr size_t kBlockWaveRunBytes = (kMaxRuns * 8 + (kMaxRuns + 1) * 4 + 15) & ~size_t(15);
+// Block-wide batch of up to BLOCK runs of one side of an item, one run per
+// thread: run starts and the exclusive prefix of their lengths.
+struct BlockRuns {
+  uint64_t* start;   // [BLOCK]
+  uint32_t* prefix;  // [BLOCK]
+};
+
+// Publishes runs r0 .. r0 + BLOCK - 1 (those below nruns > r0) of item
+// (p, q) and returns their tuple count.  Ends with a barrier.
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint32_t publish_block_runs(const PipelineDevice& d, const PipelineSide& s, uint32_t p,
+                                                       uint32_t q, uint32_t r0, uint32_t nruns, const BlockRuns& br,
+                                                       uint32_t* wave_sums) {
+  const uint32_t tid = threadIdx.x, w = tid / 64;
+  const uint32_t r = r0 + tid;
+  const LaneRun x = finish_run_at(d, raw_run_at(d, s, p, q, min(r, nruns - 1)), r < nruns);
+  const uint32_t incl = wave_inclusive_scan(x.len);
+  if (lane_id() == 63) wave_sums[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < BLOCK / 64; ++i) {
+    const uint32_t v = wave_sums[i];
+    before += i < w ? v : 0u;
+    total += v;
+  }
+  br.start[tid] = x.start;
+  br.prefix[tid] = before + incl - x.len;
+  __syncthreads();
+  return total;
+}
+
+// Tuple index of element e of a published block batch: the last run i with
+// prefix[i] <= e (empty runs share their successor's prefix and are passed).
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint64_t block_element(const BlockRuns& br, uint32_t e) {
+  uint32_t l = 0;
+#pragma unroll
+  for (uint32_t step = BLOCK / 2; step > 0; step >>= 1) l = br.prefix[l + step] <= e ? l + step : l;
+  return br.start[l] + (e - br.prefix[l]);
+}
 
 // Joins the path-0 work units, one block per unit: the item's whole build
 // side (at most CAP * 3 / 4 tuples) goes into a block-shared LDS table, then
-// the unit's probe runs are probed against it, each wave taking batches of
-// 64 runs and streaming their tuples SPL per lane at a time.  A block of
-// WAVES waves per unit keeps enough probe loads in flight where one wave per
-// unit (join_general_kernel) waits on each batch.
-template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t SPL>
-__global__ __launch_bounds__(WAVES * 64) void join_block_kernel(PipelineDevice d) {
+// the unit's probe tuples are streamed against it.  Work is split by tuples,
+// not runs: a batch of up to BLOCK runs is published block-wide and every
+// thread takes elements tid, tid + BLOCK, ... of it, SPL loads in flight, so
+// a unit of a few long runs (a heavy key, a small build side against a long
+// probe side) keeps every wave busy.
+template <typename K, uint32_t CAP, uint32_t BLOCK, uint32_t SPL>
+__global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
   using T = Tuple<K>;
   extern __shared__ uint64_t lds_u64[];
   __shared__ uint32_t s_unit;
+  __shared__ uint32_t wave_sums[BLOCK / 64];
   K* const tkeys = reinterpret_cast<K*>(lds_u64);
   const WaveTable<K, K, CAP> tab{tkeys, tkeys + CAP};
-  uint8_t* const runs = reinterpret_cast<uint8_t*>(tkeys + 2 * CAP) + (threadIdx.x / 64) * kBlockWaveRunBytes;
-  const WaveRuns wr{reinterpret_cast<uint64_t*>(runs), reinterpret_cast<uint32_t*>(runs + kMaxRuns * 8)};
-  const uint32_t w = threadIdx.x / 64, lane = lane_id();
+  const BlockRuns br{reinterpret_cast<uint64_t*>(tkeys + 2 * CAP),
+                     reinterpret_cast<uint32_t*>(reinterpret_cast<uint64_t*>(tkeys + 2 * CAP) + BLOCK)};
+  const uint32_t tid = threadIdx.x;
   const uint32_t F2 = d.bits2 ? d.F2 : 1;
   const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
   const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
@@ -640,15 +689,15 @@ __global__ __launch_bounds__(WAVES * 64) void join_block_kernel(PipelineDevice d
   const uint32_t n = uniform_load(d.work_counter + kOverflowAny) ? uniform_load(d.work_counter + kUnitCount) : 0;
   auto clear = [&]() {
     uint4* k4 = reinterpret_cast<uint4*>(tkeys);
-    for (uint32_t i = threadIdx.x; i < CAP * sizeof(K) / 16; i += WAVES * 64) k4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    for (uint32_t i = tid; i < CAP * sizeof(K) / 16; i += BLOCK) k4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
   };
   if (n) {
     clear();
     for (;;) {
-      if (threadIdx.x == 0) s_unit = atomicAdd(d.work_counter + kBlockTicket, 1u);
+      if (tid == 0) s_unit = atomicAdd(d.work_counter + kBlockTicket, 1u);
       __syncthreads();  // also orders the clear before the inserts
       const uint32_t u = s_unit;
-      __syncthreads();  // every wave has read s_unit
+      __syncthreads();  // every thread has read s_unit
       if (u >= n) break;
       const uint4 unit = d.units[u];
       if (__builtin_amdgcn_readfirstlane(unit.w) != 0) continue;  // wave-table unit
@@ -656,39 +705,36 @@ __global__ __launch_bounds__(WAVES * 64) void join_block_kernel(PipelineDevice d
       const uint32_t p = item / F2, q = item % F2;
       const uint32_t pr0 = __builtin_amdgcn_readfirstlane(unit.y), pr1 = __builtin_amdgcn_readfirstlane(unit.z);
       const uint32_t nb = run_count(d, d.build, p);
-      for (uint32_t r0 = w * kMaxRuns; r0 < nb; r0 += WAVES * kMaxRuns) {
-        const uint32_t tb = publish_runs(lane_run(d, d.build, p, q, r0, nb), wr);
-        wave_lds_sync();
-        for (uint32_t e0 = 0; e0 < tb; e0 += 4 * 64) {
-          T t[4];
+      for (uint32_t r0 = 0; r0 < nb; r0 += BLOCK) {
+        const uint32_t tb = publish_block_runs<BLOCK>(d, d.build, p, q, r0, nb, br, wave_sums);
+        for (uint32_t e0 = 0; e0 < tb; e0 += 2 * BLOCK) {
+          T t[2];
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k) t[k] = load_tuple<K>(rel_b + batch_element(wr, min(e0 + k * 64 + lane, tb - 1)));
+          for (uint32_t k = 0; k < 2; ++k)
+            t[k] = load_tuple<K>(rel_b + block_element<BLOCK>(br, min(e0 + k * BLOCK + tid, tb - 1)));
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k)
-            if (e0 + k * 64 + lane < tb && t[k].key != null_key<K>()) tab.insert(t[k].key, t[k].value);
+          for (uint32_t k = 0; k < 2; ++k)
+            if (e0 + k * BLOCK + tid < tb && t[k].key != null_key<K>()) tab.insert(t[k].key, t[k].value);
         }
-        wave_lds_sync();
+        __syncthreads();  // inserts done; the batch may be overwritten
       }
-      __syncthreads();
-      for (uint32_t r0 = pr0 + w * kMaxRuns; r0 < pr1; r0 += WAVES * kMaxRuns) {
-        const uint32_t tp = publish_runs(lane_run(d, d.probe, p, q, r0, pr1), wr);
-        wave_lds_sync();
-        for (uint32_t e0 = 0; e0 < tp; e0 += SPL * 64) {
+      for (uint32_t r0 = pr0; r0 < pr1; r0 += BLOCK) {
+        const uint32_t tp = publish_block_runs<BLOCK>(d, d.probe, p, q, r0, pr1, br, wave_sums);
+        for (uint32_t e0 = 0; e0 < tp; e0 += SPL * BLOCK) {
           T t[SPL];
 #pragma unroll
           for (uint32_t k = 0; k < SPL; ++k)
-            t[k] = load_tuple<K>(rel_p + batch_element(wr, min(e0 + k * 64 + lane, tp - 1)));
+            t[k] = load_tuple<K>(rel_p + block_element<BLOCK>(br, min(e0 + k * BLOCK + tid, tp - 1)));
 #pragma unroll
           for (uint32_t k = 0; k < SPL; ++k)
-            if (e0 + k * 64 + lane < tp && t[k].key != null_key<K>()) tab.probe(t[k].key, t[k].value, cnt, sb, sp);
+            if (e0 + k * BLOCK + tid < tp && t[k].key != null_key<K>()) tab.probe(t[k].key, t[k].value, cnt, sb, sp);
         }
-        wave_lds_sync();
+        __syncthreads();  // probes done; the batch (and then the table) may be overwritten
       }
-      __syncthreads();
       clear();
     }
   }
-  join_block_aggregate<WAVES>(d, cnt, sb, sp);
+  join_block_aggregate<BLOCK / 64>(d, cnt, sb, sp);
 }
 
 // One wave per work item (p, q).  Items are handed out in tickets of up to 8
@@ -926,18 +972,18 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
     return hipGetLastError();
   }
-  // block-shared tables: 64 KB, two blocks of 8 waves per CU
+  // block-shared tables: 64 KB, two blocks of 512 threads per CU
   constexpr uint32_t BCAP = 65536 / (2 * sizeof(K));
-  constexpr uint32_t BWAVES = 8;
+  constexpr uint32_t BBLOCK = 512;
   hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(256), 0, stream, d, BCAP * 3 / 4);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   {
-    auto blk = join_block_kernel<K, BCAP, BWAVES, 8>;
-    const size_t lds = BCAP * 2 * sizeof(K) + BWAVES * kBlockWaveRunBytes;
+    auto blk = join_block_kernel<K, BCAP, BBLOCK, 8>;
+    const size_t lds = BCAP * 2 * sizeof(K) + BBLOCK * (8 + 4);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(blk), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(lds));
-    hipLaunchKernelGGL(blk, dim3(grid), dim3(BWAVES * 64), lds, stream, d);
+    hipLaunchKernelGGL(blk, dim3(grid), dim3(BBLOCK), lds, stream, d);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
