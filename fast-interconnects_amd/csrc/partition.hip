@@ -161,9 +161,11 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
       // the loads in flight behind them.
       asm volatile("" ::"v"(pay[k]));
     }
-    // A: rank
+    // A: rank (one LDS atomic per tuple: with 2^9 partitions the lanes of a
+    // wave rarely share a digit, and a shared one only serialises that
+    // address)
 #pragma unroll
-    for (int k = 0; k < V; ++k) r[k] = wave_rank_add(cnt, d[k], act[k]);
+    for (int k = 0; k < V; ++k) r[k] = act[k] ? atomicAdd(&cnt[d[k]], 1u) : 0u;
     if (tid == 0) qcount = 0;
     __syncthreads();
     // B: place

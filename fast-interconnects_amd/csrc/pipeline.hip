@@ -543,6 +543,37 @@ constexpr uint64_t kUnitTuples = 65536;
 // by join_general_kernel's wave tables with refills (path 1).  Waves take
 // tickets of 64 items and clear the flags they find, so the flags are all
 // zero again for the next join.
+// Runs and tuples of one side of item (p, q) for the calling lane (p and q
+// may differ between lanes): tuples = the sum of the item's run lengths.
+__device__ __forceinline__ void lane_item_side(const PipelineDevice& d, const PipelineSide& s, uint32_t p,
+                                               uint32_t q, bool live, uint32_t& runs, uint64_t& tuples) {
+  runs = 0;
+  tuples = 0;
+  if (!d.bits2) {
+    const uint32_t plen = live ? s.plen[p] : 0u;
+    runs = (plen + d.tile2 - 1) / d.tile2;
+    tuples = plen;
+    return;
+  }
+  const uint32_t t0 = s.tile_start[p];
+  runs = live ? s.tile_start[p + 1] - t0 : 0u;
+  uint32_t most = runs;
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), off)));
+  // coalesced across lanes where they share p: lane l reads word q of each row
+  for (uint32_t r = 0; r < most; r += 8) {
+    uint32_t w[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      const uint32_t rr = min(r + k, runs ? runs - 1 : 0u);  // rows of an empty item: its spare row
+      w[k] = s.meta[static_cast<uint64_t>(t0 + rr) * d.F2 + q];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+      if (r + k < runs) tuples += (w[k] >> 16) - (w[k] & 0xffffu);
+  }
+}
+
 __global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d, uint32_t block_rcap) {
   if (uniform_load(d.work_counter + kOverflowAny) == 0) return;
   const uint32_t lane = lane_id();
@@ -553,32 +584,27 @@ __global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d, uint3
     if (lane == 0) t = atomicAdd(d.work_counter + kOverflowTicket, 1u);
     t = __builtin_amdgcn_readfirstlane(t);
     if (static_cast<uint64_t>(t) * 64 >= items) break;
-    const uint32_t i = t * 64 + lane;
-    const bool flagged = i < items && d.overflow[i] != 0;
-    uint64_t m = __builtin_amdgcn_ballot_w64(flagged);
-    if (flagged) d.overflow[i] = 0;
-    while (m) {
-      const uint32_t item = t * 64 + static_cast<uint32_t>(__builtin_ctzll(m));
-      m &= m - 1;
-      const uint32_t p = item / F2, q = item % F2;
-      const uint32_t np = run_count(d, d.probe, p);
-      const uint32_t nb = run_count(d, d.build, p);
-      uint64_t tuples = 0, btuples = 0;
-      for (uint32_t r0 = 0; r0 < np; r0 += kMaxRuns) tuples += lane_run(d, d.probe, p, q, r0, np).len;
-      for (uint32_t r0 = 0; r0 < nb; r0 += kMaxRuns) btuples += lane_run(d, d.build, p, q, r0, nb).len;
-      tuples = wave_broadcast_last(wave_inclusive_scan(tuples));
-      btuples = wave_broadcast_last(wave_inclusive_scan(btuples));
-      const uint32_t path = btuples <= block_rcap ? 0u : 1u;
-      uint32_t units = static_cast<uint32_t>(min<uint64_t>((tuples + kUnitTuples - 1) / kUnitTuples, np));
-      units = max(units, 1u);
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(d.work_counter + kUnitCount, units);
-      base = __builtin_amdgcn_readfirstlane(base);
-      for (uint32_t u = lane; u < units; u += 64) {
-        const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(u) * np / units);
-        const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(u + 1) * np / units);
-        d.units[base + u] = make_uint4(item, r0, r1, path);
-      }
+    // one item per lane: its two sides' run counts and tuple counts
+    const uint32_t item = min(t * 64 + lane, items - 1);
+    const bool flagged = t * 64 + lane < items && d.overflow[item] != 0;
+    if (__builtin_amdgcn_ballot_w64(flagged) == 0) continue;
+    if (flagged) d.overflow[item] = 0;
+    const uint32_t p = item / F2, q = item % F2;
+    uint32_t np, nb;
+    uint64_t tuples, btuples;
+    lane_item_side(d, d.probe, p, q, flagged, np, tuples);
+    lane_item_side(d, d.build, p, q, flagged, nb, btuples);
+    const uint32_t path = btuples <= block_rcap ? 0u : 1u;
+    uint32_t units = 0;
+    if (flagged) units = max(1u, static_cast<uint32_t>(min<uint64_t>((tuples + kUnitTuples - 1) / kUnitTuples, np)));
+    const uint32_t incl = wave_inclusive_scan(units);
+    uint32_t base = 0;
+    if (lane == 63) base = atomicAdd(d.work_counter + kUnitCount, incl);
+    base = __builtin_amdgcn_readlane(base, 63) + incl - units;
+    for (uint32_t u = 0; u < units; ++u) {
+      const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(u) * np / units);
+      const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(u + 1) * np / units);
+      d.units[base + u] = make_uint4(t * 64 + lane, r0, r1, path);
     }
   }
 }
@@ -979,7 +1005,7 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   {
-    auto blk = join_block_kernel<K, BCAP, BBLOCK, 8>;
+    auto blk = join_block_kernel<K, BCAP, BBLOCK, 16>;
     const size_t lds = BCAP * 2 * sizeof(K) + BBLOCK * (8 + 4);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(blk), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(lds));
