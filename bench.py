@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--cpu-log2n", type=int, default=23, help="CPU baseline sample size (log2 of |R| = |S|)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--verify", action="store_true", help="check the result against the oracle at full size")
-    ap.add_argument("--workload", choices=["plain", "skew", "batched", "served", "ooc"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "skew", "batched", "served", "ooc", "np"], default="plain",
                     help="plain: R = S = 2^27 int64, 9+9 bits (the headline line); skew: R = 2^27 int32 unique, "
                          "S = 2^30 int32 Zipf over R, 8+8 bits; batched: 4096 int32 joins of log-uniform sizes")
     ap.add_argument("--zipf", type=float, default=1.0, help="skew: Zipf exponent")
@@ -474,6 +474,63 @@ def ooc_main(a):
     print(json.dumps(out))
 
 
+def np_main(a):
+    """The no-partitioning join (the hashjoin binary's comparison point):
+    R = S = 2^log2n int64 tuples with the C2 generators, a global
+    linear-probing table of 2 * next_pow2(|R|) entries, grid 2 x CUs of 256
+    threads (hashjoin/src/main.rs:383-390); and the same probe side against
+    a build side of 2^22 tuples, whose table (128 MB) fits the GPU's last
+    level cache."""
+    import fij
+
+    n = 1 << a.log2n
+    grid = 2 * fij.device_limits()["compute_units"]
+    sk = fij.DeviceArray(n, np.int64)
+    sp = fij.DeviceArray(n, np.int64)
+    fij.gen_iota(sp, 0)
+    lines = {}
+    for log2r in (a.log2n, 22):
+        nr = 1 << log2r
+        rk = fij.DeviceArray(nr, np.int64)
+        fij.gen_permutation(rk, SEED, 1)
+        fij.gen_uniform(sk, SEED, 1, 1, nr)
+        fij.synchronize()
+        entries = 2 * nr
+        ht = fij.HashTable(entries, fij.KeyType.I64)
+        hj = fij.HipHashJoin(fij.HashingScheme.LinearProbing, ht, (grid, 256), (grid, 256))
+        agg = fij.aggregates_array(grid)
+        e = [fij.Event() for _ in range(3)]
+        tb, tp = [], []
+        for step in range(a.warmup + a.steps):
+            fij.check(fij.lib.fij_np_table_init(int(fij.KeyType.I64), fij._ptr(ht.mem), entries, None))
+            agg.fill(0)
+            e[0].record()
+            hj.build(rk, rk)  # payload = key
+            e[1].record()
+            hj.probe_sum(sk, sp, None, agg)
+            e[2].record()
+            fij.synchronize()
+            if step >= a.warmup:
+                tb.append(e[1].elapsed_ms(e[0]))
+                tp.append(e[2].elapsed_ms(e[1]))
+        res = fij.sum_aggregates(agg)
+        assert res[0] == n and res[2] == (n * (n - 1) // 2) % (1 << 64), "NP join result"
+        b, p_ = float(np.median(tb)), float(np.median(tp))
+        lines[f"R=2^{log2r}"] = {"build_ms": b, "probe_ms": p_, "tuples_per_s": (nr + n) / ((b + p_) / 1000.0),
+                                  "table_bytes": entries * 16}
+        del rk, ht, hj
+    first = lines[f"R=2^{a.log2n}"]
+    out = {
+        "metric": METRIC, "value": first["tuples_per_s"], "unit": "tuples/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": first["build_ms"] + first["probe_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic (seeded on-device generators)",
+        "config": {"workload": f"no-partitioning linear-probing join, R = S = 2^{a.log2n} int64, grid {grid} x 256",
+                   "parallelism": "single GPU"},
+        "lines": lines,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -488,6 +545,8 @@ def main():
         return served_main(a)
     if a.workload == "ooc":
         return ooc_main(a)
+    if a.workload == "np":
+        return np_main(a)
     import fij
 
     n = 1 << a.log2n
