@@ -143,7 +143,12 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     for (uint32_t k = 0; k < ITEMS; ++k) {
       const bool act = k * BLOCK + tid < r.n;
       dig[k] = key_to_partition<K>(v[k].key, mask2, d.bits1);
-      rank[k] = wave_rank_add(cnt, dig[k], act);
+      // 8-byte tuples: one LDS atomic per tuple is faster; 16-byte tuples
+      // (half as many per tile) gain from aggregating lanes of a digit
+      if constexpr (sizeof(K) == 4)
+        rank[k] = act ? atomicAdd(&cnt[dig[k]], 1u) : 0u;
+      else
+        rank[k] = wave_rank_add(cnt, dig[k], act);
     }
     __syncthreads();
     block_exclusive_scan<uint32_t>(cnt, F2, wave_tmp);
