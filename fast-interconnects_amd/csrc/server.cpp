@@ -303,6 +303,13 @@ fij_status fij_join_server_create(const fij_join_server_config* cfg, fij_join_se
   if (!cfg || !out || (cfg->type != FIJ_TYPE_I32 && cfg->type != FIJ_TYPE_I64) || cfg->slot_classes == 0 ||
       cfg->slot_classes > FIJ_SERVER_MAX_SLOT_CLASSES)
     return server_fail(FIJ_INVALID_ARGUMENT, "invalid server configuration");
+  uint32_t total_slots = 0;
+  for (uint32_t c = 0; c < cfg->slot_classes; ++c) {
+    if (cfg->slots[c] && cfg->slot_probe_len[c] == 0)
+      return server_fail(FIJ_INVALID_ARGUMENT, "slot class %u holds no probe tuples", c);
+    total_slots += cfg->slots[c];
+  }
+  if (total_slots == 0) return server_fail(FIJ_INVALID_ARGUMENT, "a server needs at least one slot");
   int dev = 0;
   SRV_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -317,9 +324,6 @@ fij_status fij_join_server_create(const fij_join_server_config* cfg, fij_join_se
   s->lds_total = static_cast<uint32_t>(prop.sharedMemPerBlock);
   if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > 0)
     s->lds_total = std::max<uint32_t>(s->lds_total, static_cast<uint32_t>(optin));
-  uint32_t total_slots = 0;
-  for (uint32_t c = 0; c < cfg->slot_classes; ++c) total_slots += cfg->slots[c];
-  if (total_slots == 0) return server_fail(FIJ_INVALID_ARGUMENT, "a server needs at least one slot");
   s->slots.resize(total_slots);
   uint32_t k = 0;
   fij_status st = FIJ_OK;

@@ -280,3 +280,42 @@ def test_np_build_table_too_small():
     rc = fij.lib.fij_np_build(1, 1, 0, C.c_void_p(FAKE), 16, C.c_void_p(FAKE), C.c_void_p(FAKE), 17, 1, 256, None)
     assert rc == Status.INVALID_ARGUMENT
     assert "too small" in fij.lib.fij_last_error().decode()
+
+
+# ---- host-side rules and argument checks of the batch, server and
+# out-of-core entry points (no device work)
+
+def test_autotune_rule():
+    import math
+    assert fij.autotune(np.int32, 1000, 16000)[0]          # one block's LDS table
+    assert not fij.autotune(np.int32, 12289, 16000)[0]      # build side beyond one table fill
+    assert not fij.autotune(np.int32, 1000, (1 << 22) + 1)[0]
+    assert fij.autotune(np.int64, 6144, 1 << 20)[0] and not fij.autotune(np.int64, 6145, 1 << 20)[0]
+    small, b1, b2 = fij.autotune(np.int64, 1 << 27, 1 << 27)
+    assert not small and (b1, b2) == (9, 9)
+    for n_r in (1 << 14, 1 << 18, 3_000_000, 1 << 22, 1 << 26):
+        for f in (1, 4, 16):
+            small, b1, b2 = fij.autotune(np.int32, n_r, n_r * f)
+            assert 1 <= b1 <= 9 and 1 <= b2 <= 12
+            # at most 512 build tuples per final partition on average
+            assert n_r >> (b1 + b2) <= 512 or (b1, b2) == (9, 12)
+            # a pass-1 partition's probe side spans at most 64 pass-2 tiles of 8192
+            assert (n_r * f) >> b1 <= 64 * 8192 or b1 == 9
+
+
+def test_server_config_rejected_on_host():
+    with pytest.raises(fij.FijError) as e:
+        fij.JoinServer(np.int64, [(0, 1 << 20)], max_probe_len=1 << 20)  # no slot at all
+    assert e.value.status == fij.Status.INVALID_ARGUMENT
+    with pytest.raises(ValueError):
+        fij.JoinServer(np.int64, [(1, 1 << 10)] * 5, max_probe_len=1 << 20)  # more classes than the ABI holds
+
+
+def test_ooc_arguments_rejected_on_host():
+    k = np.arange(1, 1001, dtype=np.int64)
+    for bits in ((0, 4), (4, 0), (13, 4), (4, 13)):
+        with pytest.raises(fij.FijError) as e:
+            fij.ooc_join(np.int64, k, k, k, k, *bits, pass1_cpu=True)
+        assert e.value.status == fij.Status.INVALID_ARGUMENT
+    with pytest.raises(fij.FijError):
+        fij.ooc_join(np.int64, k[1:].view(np.int32)[1:1000], k[:999], k, k, 4, 4, pass1_cpu=True)  # misaligned

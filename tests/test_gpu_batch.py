@@ -65,12 +65,6 @@ def test_batch_null_keys_and_duplicates():
     assert got == [orc.sort_merge_join(*h) for h in host]
 
 
-def test_autotune_rule():
-    assert fij.autotune(np.int32, 1000, 16000)[0]
-    small, b1, b2 = fij.autotune(np.int64, 1 << 27, 1 << 27)
-    assert not small and (b1, b2) == (9, 9)
-    small, b1, b2 = fij.autotune(np.int32, 1 << 22, 1 << 26)
-    assert not small and b1 >= 7 and (1 << 22) >> (b1 + b2) <= 640
 
 
 def test_plan_enqueue_sized_matches_execute():
