@@ -7,22 +7,27 @@
 // partition is not defined (as in the reference).  Padding in front of each
 // partition is never written.
 //
-// The buffered variant is designed for wave64 / 160 KB LDS: instead of the
-// reference's per-buffer spin locks it processes the chunk in block-wide
-// tiles with three barrier-separated phases, so no wave ever waits on
-// another wave's progress inside a phase:
-//   A  rank:   wave-aggregated LDS atomics give each tuple its position in
-//              its partition's output sequence (ballot + mbcnt);
-//   B  place:  the tuple goes into its partition's LDS ring of R aligned
-//              128-byte lines; a tuple that completes a line enqueues it
-//              (with the partition's low-water mark at that time).  A tuple
-//              beyond the ring's capacity is stored directly;
-//   C  flush + commit: waves write queued lines, 64/L lines per store
-//              instruction, each a full aligned 128-byte line, while the
-//              per-partition fill / flushed / low-water marks advance.
-// Positions below a partition's low-water mark (front alignment slots and
-// slots already stored directly) are never written from the ring.  The next
-// tile's loads are in flight while a tile goes through the phases.
+// The buffered variant is designed for wave64 / 160 KB LDS.  A block's waves
+// have one of two roles, so that no wave both loads and stores global
+// memory: on gfx950 one counter (vmcnt) tracks a wave's loads and stores in
+// issue order, and a wave that waits for a prefetched load would also wait
+// for every store it issued before that load.
+//   loaders:  load the chunk's key and payload columns (16 bytes per lane
+//             and column) eight steps ahead, take each tuple's position
+//             with one 64-bit LDS atomic on its partition's {fill, limit}
+//             word, and write the tuple into the partition's LDS ring of R
+//             aligned 128-byte lines; a tuple that completes a line queues
+//             the line (one queue reservation per wave and step);
+//   flushers: write the lines queued in the previous step, 64/L lines per
+//             store instruction, each a full aligned 128-byte line.
+// Steps are separated by two barriers (a step's placement and the flush of
+// the previous step's lines, then a commit of the per-partition marks).  A
+// line is read by the flushers one step after it was completed, so the ring
+// slots a step may fill end RL positions after the first line that was
+// still unflushed when the previous step began; a tuple beyond that limit is
+// stored directly (rare unless a partition receives more than about RL/2
+// tuples per step) and positions below a partition's low-water mark are
+// never written from the ring.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -39,6 +44,7 @@ struct ScatterParams {
   const uint64_t* chunk_offsets;
   void* relation;
   uint32_t ring_lines;
+  uint32_t queue_len;    // line queue entries per step buffer
   uintptr_t spill_addr;  // address tuple index 0 would have in the spill region
   uint32_t spill_from;   // first partition written to the spill region
 };
@@ -89,201 +95,284 @@ __global__ __launch_bounds__(1024) void scatter_direct_kernel(ScatterParams p) {
   }
 }
 
-template <typename K, int BLOCK, int V, bool SPILL>
+// Steps whose loads a loader has in flight.
+constexpr uint32_t kScatterPrefetch = 8;
+
+// V keys (and V payloads) per loader lane and step: one 16-byte load each.
+template <typename K>
+struct Vec16 {
+  typedef K type __attribute__((ext_vector_type(16 / sizeof(K))));
+};
+
+template <typename K, int BLOCK, bool SPILL>
 __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   using T = Tuple<K>;
+  using VK = typename Vec16<K>::type;
+  constexpr uint32_t V = 16 / sizeof(K);          // tuples per loader lane and step
   constexpr uint32_t L = kAlignBytes / sizeof(T);  // tuples per 128-byte line
-  constexpr uint32_t TILE = BLOCK * V;
   constexpr uint32_t NW = BLOCK / kWave;
-  constexpr uint32_t LPW = kWave / L;              // lines per wave store
-  extern __shared__ uint64_t lds_u64[];
-  __shared__ uint32_t qcount;
+  constexpr uint32_t NLW = NW / 2;                 // loader waves
+  constexpr uint32_t NFW = NW - NLW;               // flusher waves
+  constexpr uint32_t NLT = NLW * kWave;            // loader lanes
+  constexpr uint32_t LPW = kWave / L;              // lines per flusher store
+  constexpr uint32_t UF = L / 4;                   // flusher stores per round
+  constexpr uint32_t D = kScatterPrefetch;         // steps loaded ahead
+  extern __shared__ __attribute__((aligned(16))) uint64_t lds_u64[];
+  __shared__ uint32_t qcount[2];
 
   const uint32_t F = 1u << p.radix_bits;
   const uint32_t R = p.ring_lines;
   const uint32_t RL = R * L;
+  const uint32_t QL = p.queue_len;
   const uint64_t mask = radix_mask(p.radix_bits, p.ignore_bits);
   const uint32_t tid = threadIdx.x;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const uint32_t lane = lane_id();
 
-  T* ring = reinterpret_cast<T*>(lds_u64);            // [F][RL]
-  uint32_t* base = reinterpret_cast<uint32_t*>(ring + static_cast<size_t>(F) * RL);
-  uint32_t* fill = base + F;      // next position to assign (counts front alignment)
-  uint32_t* flushed = fill + F;   // first position not yet written out (line aligned)
-  uint32_t* ring_lo = flushed + F;  // positions below are never written from the ring
-  uint32_t* cnt = ring_lo + F;    // per-tile arrivals
-  uint32_t* qpart = cnt + F;      // [F * R] queued line: partition
-  uint32_t* qline = qpart + static_cast<size_t>(F) * R;  // [F * R] queued line: line number
-  uint32_t* qlo = qline + static_cast<size_t>(F) * R;    // [F * R] queued line: partition's low-water mark
+  // Positions are tuple indices from `org`, a 128-byte aligned address at or
+  // below the chunk's first partition: a position is line aligned when it is
+  // a multiple of L, and a partition's ring slot for position P is P mod RL.
+  T* ring = reinterpret_cast<T*>(lds_u64);  // [F][RL]
+  uint2* queue = reinterpret_cast<uint2*>(ring + static_cast<size_t>(F) * RL);  // [2][QL] {partition, line}
+  uint64_t* fl = reinterpret_cast<uint64_t*>(queue + 2 * static_cast<size_t>(QL));  // {fill, lim}
+  uint32_t* fprev = reinterpret_cast<uint32_t*>(fl + F);  // fill at the end of the previous step
+  uint32_t* ring_lo = fprev + F;       // positions below are never written from the ring
+  uint32_t* ring_lo_prev = ring_lo + F;  // ring_lo during the previous step
 
   const uint64_t* off = p.chunk_offsets + static_cast<uint64_t>(blockIdx.x) * F;
-  // The first partition starts at least one padding (= one line) into the
-  // chunk's range, so cbase keeps every aligned line start non-negative.
-  const uint64_t cbase = off[0] - L;
-  T* rel = static_cast<T*>(p.relation) + cbase;
+  const uint64_t cbase = off[0];
+  T* const rel = static_cast<T*>(p.relation) + cbase;
+  const uint32_t a = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(rel) % kAlignBytes) / sizeof(T));
+  // Destination of partition d's position P: org(d) + P.
+  T* const org_rel = rel - a;
+  T* const org_spill = SPILL ? reinterpret_cast<T*>(p.spill_addr) + cbase - a : org_rel;
+  auto org = [&](uint32_t d) -> T* {
+    if constexpr (SPILL) return d >= p.spill_from ? org_spill : org_rel;
+    return org_rel;
+  };
   for (uint32_t i = tid; i < F; i += BLOCK) {
-    const uint32_t o = static_cast<uint32_t>(off[i] - cbase);
-    const uint32_t m = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(rel + o) % kAlignBytes) / sizeof(T));
-    base[i] = o - m;
-    fill[i] = m;
-    flushed[i] = 0;
-    ring_lo[i] = m;
-    cnt[i] = 0;
+    const uint32_t s0 = static_cast<uint32_t>(off[i] - cbase) + a;
+    fl[i] = static_cast<uint64_t>(s0) | (static_cast<uint64_t>((s0 & ~(L - 1)) + RL) << 32);
+    fprev[i] = s0;
+    ring_lo[i] = s0;
+    ring_lo_prev[i] = s0;
   }
-  if (tid == 0) qcount = 0;
+  if (tid < 2) qcount[tid] = 0;
   __syncthreads();
 
   uint64_t begin, len;
   chunk_range<K>(p, begin, len);
   const K* keys = static_cast<const K*>(p.keys) + begin;
   const K* pays = static_cast<const K*>(p.pays) + begin;
+  const uint64_t nvec = len / V;
 
-  // Tile t0's keys and payloads; lanes past the chunk's end re-read its last
-  // element, so every tile issues the same unconditional loads and the next
-  // tile's loads stay in flight while this tile is ranked, placed and flushed.
-  auto load_tile = [&](K (&key)[V], K (&pay)[V], uint64_t t0) {
+  // Loader: places up to V tuples of one lane (act: bit k = tuple k is live)
+  // and queues the lines they complete into queue buffer qb.  One LDS round
+  // trip: the 64-bit atomic returns the position and the step's limit.
+  auto place = [&](const VK& key, const VK& pay, uint32_t act, uint32_t qb) {
+    uint32_t d[V], pos[V], lm[V];
 #pragma unroll
-    for (int k = 0; k < V; ++k) {
-      const uint64_t e = min(t0 + static_cast<uint64_t>(k) * BLOCK + tid, len - 1);
-      key[k] = __builtin_nontemporal_load(keys + e);
-      pay[k] = __builtin_nontemporal_load(pays + e);
-    }
-  };
-  auto process = [&](const K (&key)[V], const K (&pay)[V], uint64_t t0) {
-    uint32_t d[V], r[V];
-    bool act[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-      act[k] = t0 + static_cast<uint64_t>(k) * BLOCK + tid < len;
+    for (uint32_t k = 0; k < V; ++k) {
       d[k] = key_to_partition<K>(key[k], mask, p.ignore_bits);
-      // The payloads are used on some paths only; using them here on every
-      // path retires their loads in this tile's one wait, so the next
-      // tile's loads (issued into the same registers later) never wait for
-      // the loads in flight behind them.
-      asm volatile("" ::"v"(pay[k]));
+      pos[k] = 0;
+      lm[k] = 0;
+      if ((act >> k) & 1u) {
+        const uint64_t o = atomicAdd(reinterpret_cast<unsigned long long*>(&fl[d[k]]), 1ull);
+        pos[k] = static_cast<uint32_t>(o);
+        lm[k] = static_cast<uint32_t>(o >> 32);
+      }
     }
-    // A: rank (one LDS atomic per tuple: with 2^9 partitions the lanes of a
-    // wave rarely share a digit, and a shared one only serialises that
-    // address)
+    uint64_t cm[V];
+    uint32_t total = 0;
 #pragma unroll
-    for (int k = 0; k < V; ++k) r[k] = act[k] ? atomicAdd(&cnt[d[k]], 1u) : 0u;
-    if (tid == 0) qcount = 0;
-    __syncthreads();
-    // B: place
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
+    for (uint32_t k = 0; k < V; ++k) {
       bool complete = false;
-      uint32_t pos = 0;
-      if (act[k]) {
-        const uint32_t f = fill[d[k]];
-        const uint32_t fl = flushed[d[k]];
-        pos = f + r[k];
-        if (pos < fl + RL) {
+      if ((act >> k) & 1u) {
+        if (pos[k] < lm[k]) {
           T t;
           t.key = key[k];
           t.value = pay[k];
-          ring[static_cast<size_t>(d[k]) * RL + (pos & (RL - 1))] = t;
-          complete = (pos & (L - 1)) == L - 1;
+          ring[static_cast<size_t>(d[k]) * RL + (pos[k] & (RL - 1))] = t;
+          complete = (pos[k] & (L - 1)) == L - 1;
         } else {
-          store_tuple<K>(part_dst<SPILL>(p, rel, cbase, d[k]) + base[d[k]] + pos, key[k], pay[k]);
+          store_tuple<K>(org(d[k]) + pos[k], key[k], pay[k]);
         }
       }
-      const uint64_t cm = ballot(complete);
-      if (cm) {
-        const int leader = __ffsll(static_cast<unsigned long long>(cm)) - 1;
-        uint32_t qb = 0;
-        if (static_cast<int>(lane_id()) == leader) qb = atomicAdd(&qcount, static_cast<uint32_t>(__popcll(cm)));
-        qb = __builtin_amdgcn_readlane(qb, leader);
-        if (complete) {
-          const uint32_t q = qb + mbcnt(cm);
-          qpart[q] = d[k];
-          qline[q] = pos / L;
-          qlo[q] = ring_lo[d[k]];
-        }
+      cm[k] = ballot(complete);
+      total += static_cast<uint32_t>(__popcll(cm[k]));
+    }
+    if (total) {
+      uint32_t q0 = 0;
+      if (lane == 0) q0 = atomicAdd(&qcount[qb], total);
+      q0 = __builtin_amdgcn_readfirstlane(q0) + qb * QL;
+#pragma unroll
+      for (uint32_t k = 0; k < V; ++k) {
+        if ((cm[k] >> lane) & 1u) queue[q0 + mbcnt(cm[k])] = make_uint2(d[k], pos[k] & ~(L - 1));
+        q0 += static_cast<uint32_t>(__popcll(cm[k]));
       }
     }
-    __syncthreads();
-    // C: flush full lines (reads ring, the queue and base only) and commit
-    // (writes fill, flushed, ring_lo and cnt only)
-    {
-      const uint32_t nq = qcount;
-      const uint32_t w = tid / kWave;
-      const uint32_t lane = lane_id();
-      for (uint32_t q0 = w * LPW; q0 < nq; q0 += NW * LPW) {
-        const uint32_t q = q0 + lane / L;
-        if (q < nq) {
-          const uint32_t pp = qpart[q];
-          const uint32_t pos = qline[q] * L + (lane % L);
-          if (pos >= qlo[q]) {
-            const T t = ring[static_cast<size_t>(pp) * RL + (pos & (RL - 1))];
-            store_tuple<K>(part_dst<SPILL>(p, rel, cbase, pp) + base[pp] + pos, t.key, t.value);
-          }
-        }
+  };
+  // Flusher: writes the lines of queue buffer qb, UF stores per round.
+  auto flush = [&](uint32_t qb) {
+    const uint32_t nq = qcount[qb];
+    const uint32_t j = lane % L;
+    for (uint32_t q0 = (w - NLW) * LPW * UF; q0 < nq; q0 += NFW * LPW * UF) {
+      uint2 e[UF];
+      bool live[UF];
+#pragma unroll
+      for (uint32_t u = 0; u < UF; ++u) {
+        const uint32_t q = q0 + u * LPW + lane / L;
+        live[u] = q < nq;
+        e[u] = live[u] ? queue[qb * QL + q] : make_uint2(0, 0);
       }
+      T t[UF];
+      uint32_t lo[UF];
+#pragma unroll
+      for (uint32_t u = 0; u < UF; ++u) {
+        lo[u] = ring_lo_prev[e[u].x];
+        t[u] = ring[static_cast<size_t>(e[u].x) * RL + ((e[u].y + j) & (RL - 1))];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < UF; ++u)
+        if (live[u] && e[u].y + j >= lo[u]) store_tuple<K>(org(e[u].x) + e[u].y + j, t[u].key, t[u].value);
     }
+  };
+  // Between steps: the limit of the next step, the low-water mark after a
+  // direct store, and the flushed queue buffer emptied for reuse.
+  auto commit = [&](uint32_t qb_flushed) {
     for (uint32_t i = tid; i < F; i += BLOCK) {
-      const uint32_t c = cnt[i];
-      if (c) {
-        const uint32_t nf = fill[i] + c;
-        if (nf > flushed[i] + RL) ring_lo[i] = nf;
-        flushed[i] = nf & ~(L - 1);
-        fill[i] = nf;
-        cnt[i] = 0;
-      }
+      const uint64_t v = fl[i];
+      const uint32_t f = static_cast<uint32_t>(v);
+      const uint32_t fp = fprev[i];
+      const uint32_t lo = ring_lo[i];
+      ring_lo_prev[i] = lo;
+      if (f > static_cast<uint32_t>(v >> 32)) ring_lo[i] = f;
+      reinterpret_cast<uint32_t*>(&fl[i])[1] = (fp & ~(L - 1)) + RL;
+      fprev[i] = f;
     }
-    __syncthreads();
+    if (tid == 0) qcount[qb_flushed] = 0;
   };
 
-  if (len) {
-    // Three register buffers, loads issued two tiles ahead, unrolled by
-    // three with no exit inside the loop: every path into the loop head has
-    // consumed buffer c, so issuing its next loads never waits for the
-    // loads in flight in a and b.  Loads past the end are clamped to the
-    // last element.
-    const uint64_t tiles = (len + TILE - 1) / TILE;
-    K ka[V], pa[V], kb[V], pb[V], kc[V], pc[V];
-    load_tile(ka, pa, 0);
-    load_tile(kb, pb, TILE);
-    uint64_t t = 0;
-    for (; t + 3 <= tiles; t += 3) {
-      load_tile(kc, pc, (t + 2) * TILE);
-      process(ka, pa, t * TILE);
-      load_tile(ka, pa, (t + 3) * TILE);
-      process(kb, pb, (t + 1) * TILE);
-      load_tile(kb, pb, (t + 4) * TILE);
-      process(kc, pc, (t + 2) * TILE);
+  // The roles run separate loops with the same barriers (two per step,
+  // three at the end), so the loaders' load waits are counted over loads
+  // only.
+  const uint64_t steps = (nvec + NLT - 1) / NLT;
+  const uint64_t steps_run = (steps + D - 1) / D * D;  // whole rounds of D
+  const uint32_t qlast = static_cast<uint32_t>(steps_run & 1);
+  if (w < NLW) {
+    const uint32_t lt = tid;  // loader lane
+    if (steps) {
+      // D register buffers, each step's loads issued D steps ahead; loads
+      // past the last full vector re-read it, so every step issues the
+      // same loads.
+      VK kv[D], pv[D];
+#pragma unroll
+      for (uint32_t b = 0; b < D; ++b) {
+        const uint64_t v = min(static_cast<uint64_t>(b) * NLT + lt, nvec - 1);
+        kv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(keys) + v);
+        pv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(pays) + v);
+        // issued in buffer order: the loop's waits count on it
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      for (uint64_t s0 = 0; s0 < steps_run; s0 += D) {
+#pragma unroll
+        for (uint32_t b = 0; b < D; ++b) {
+          const uint64_t s = s0 + b;
+          place(kv[b], pv[b], s * NLT + lt < nvec ? (1u << V) - 1 : 0u, static_cast<uint32_t>(s & 1));
+          // The buffer is reloaded only after its last use, so the loaded
+          // registers stay in place across the loop's back edge (a copy of a
+          // register with a load in flight would wait for that load).
+          __builtin_amdgcn_sched_barrier(0);
+          const uint64_t v = min((s + D) * NLT + lt, nvec - 1);
+          kv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(keys) + v);
+          pv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(pays) + v);
+          __syncthreads();
+          commit(static_cast<uint32_t>(s & 1) ^ 1u);
+          __syncthreads();
+        }
+      }
     }
-    if (t < tiles) process(ka, pa, t * TILE);
-    if (t + 1 < tiles) process(kb, pb, (t + 1) * TILE);
+    // The last len % V tuples (loader lane 0).
+    VK key, pay;
+    uint32_t act = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < V; ++k) {
+      const uint64_t e = nvec * V + k;
+      key[k] = 0;
+      pay[k] = 0;
+      if (lt == 0 && e < len) {
+        key[k] = keys[e];
+        pay[k] = pays[e];
+        act |= 1u << k;
+      }
+    }
+    place(key, pay, act, qlast);
+    __syncthreads();
+    commit(qlast ^ 1u);
+    __syncthreads();
+    __syncthreads();
+  } else {
+    for (uint64_t s = 0; s < steps_run; ++s) {
+      flush(static_cast<uint32_t>(s & 1) ^ 1u);
+      __syncthreads();
+      commit(static_cast<uint32_t>(s & 1) ^ 1u);
+      __syncthreads();
+    }
+    flush(qlast ^ 1u);
+    __syncthreads();
+    commit(qlast ^ 1u);
+    __syncthreads();
+    flush(qlast);
+    __syncthreads();
   }
   // Write the partially filled last line of every partition.
   for (uint32_t idx = tid; idx < F * L; idx += BLOCK) {
     const uint32_t i = idx / L;
-    const uint32_t pos = flushed[i] + (idx % L);
-    if (pos < fill[i] && pos >= ring_lo[i]) {
+    const uint32_t f = static_cast<uint32_t>(fl[i]);
+    const uint32_t pos = (f & ~(L - 1)) + (idx % L);
+    if (pos < f && pos >= ring_lo[i]) {
       const T t = ring[static_cast<size_t>(i) * RL + (pos & (RL - 1))];
-      store_tuple<K>(part_dst<SPILL>(p, rel, cbase, i) + base[i] + pos, t.key, t.value);
+      store_tuple<K>(org(i) + pos, t.key, t.value);
     }
   }
 }
 
-// Launches the buffered scatter with the deepest ring (4, 2 or 1 lines per
-// partition) whose LDS fits next to the kernel's static LDS; returns
-// hipErrorNotSupported without launching when even one line does not fit.
-template <typename K, int BLOCK, int V, bool SPILL>
+// Queue entries per step buffer: a step completes at most one line per
+// placed tuple and at most one line per ring line.
+template <typename K>
+uint32_t swwc_queue_len(uint32_t fanout, uint32_t ring_lines, uint32_t block) {
+  const uint64_t tile = static_cast<uint64_t>(block / 2) * (16 / sizeof(K)) + 1;
+  const uint64_t ring = static_cast<uint64_t>(fanout) * ring_lines;
+  return static_cast<uint32_t>(tile < ring ? tile : ring);
+}
+
+uint32_t swwc_bytes(uint32_t fanout, uint32_t ring_lines, uint32_t queue_len) {
+  const uint64_t bytes = static_cast<uint64_t>(fanout) * ring_lines * kAlignBytes  // rings
+                         + static_cast<uint64_t>(queue_len) * 2 * 8                // line queues
+                         + static_cast<uint64_t>(fanout) * 4 * 5;                  // per-partition words
+  return bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(bytes);
+}
+
+// Launches the buffered scatter with the deepest ring (up to 256 lines per
+// partition, so that low fanouts buffer many tuples per partition) whose LDS
+// fits next to the kernel's static LDS; returns hipErrorNotSupported
+// without launching when even one line does not fit.
+template <typename K, int BLOCK, bool SPILL>
 hipError_t launch_swwc_t(ScatterParams p, uint32_t fanout, uint32_t chunks, uint32_t lds_total,
                          hipStream_t stream) {
-  auto kern = scatter_swwc_kernel<K, BLOCK, V, SPILL>;
+  auto kern = scatter_swwc_kernel<K, BLOCK, SPILL>;
   const uint32_t budget = dynamic_lds_budget(reinterpret_cast<const void*>(kern), lds_total);
   uint32_t ring = 0;
-  for (uint32_t r = 4; r >= 1; r /= 2) {
-    if (swwc_lds_bytes(fanout, sizeof(Tuple<K>), r) <= budget) {
+  for (uint32_t r = 256; r >= 1; r /= 2) {
+    if (swwc_bytes(fanout, r, swwc_queue_len<K>(fanout, r, BLOCK)) <= budget) {
       ring = r;
       break;
     }
   }
   if (ring == 0) return hipErrorNotSupported;
   p.ring_lines = ring;
-  const uint32_t lds = swwc_lds_bytes(fanout, sizeof(Tuple<K>), ring);
+  p.queue_len = swwc_queue_len<K>(fanout, ring, BLOCK);
+  const uint32_t lds = swwc_bytes(fanout, ring, p.queue_len);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));
   hipLaunchKernelGGL(kern, dim3(chunks), dim3(BLOCK), lds, stream, p);
@@ -305,17 +394,19 @@ hipError_t launch_partition_t(const PartitionLaunch& a, hipStream_t stream) {
   const bool spill = a.spill != nullptr && a.spill_from < F;
   p.spill_from = spill ? a.spill_from : F;
   p.spill_addr = reinterpret_cast<uintptr_t>(a.spill) - static_cast<uintptr_t>(a.spill_base * sizeof(Tuple<K>));
-  if (a.buffered) {
-    constexpr int V = sizeof(K) == 4 ? 4 : 2;
+  // The buffered scatter loads 16 bytes per lane: both columns must be
+  // 16-byte aligned (chunk starts are multiples of 128 bytes from them).
+  const bool vec_ok = reinterpret_cast<uintptr_t>(a.keys) % 16 == 0 && reinterpret_cast<uintptr_t>(a.pays) % 16 == 0;
+  if (a.buffered && vec_ok) {
     hipError_t e;
     if (spill)
-      e = launch_swwc_t<K, 1024, V, true>(p, F, a.chunks, a.lds_bytes, stream);
+      e = launch_swwc_t<K, 1024, true>(p, F, a.chunks, a.lds_bytes, stream);
     else if (a.block >= 1024)
-      e = launch_swwc_t<K, 1024, V, false>(p, F, a.chunks, a.lds_bytes, stream);
+      e = launch_swwc_t<K, 1024, false>(p, F, a.chunks, a.lds_bytes, stream);
     else if (a.block >= 512)
-      e = launch_swwc_t<K, 512, V, false>(p, F, a.chunks, a.lds_bytes, stream);
+      e = launch_swwc_t<K, 512, false>(p, F, a.chunks, a.lds_bytes, stream);
     else
-      e = launch_swwc_t<K, 256, V, false>(p, F, a.chunks, a.lds_bytes, stream);
+      e = launch_swwc_t<K, 256, false>(p, F, a.chunks, a.lds_bytes, stream);
     if (e != hipErrorNotSupported) return e;
   }
   // Direct scatter: per-partition cursors only (4 bytes per partition).
@@ -330,11 +421,9 @@ hipError_t launch_partition_t(const PartitionLaunch& a, hipStream_t stream) {
 }  // namespace
 
 uint32_t swwc_lds_bytes(uint32_t fanout, uint32_t tuple_bytes, uint32_t ring_lines) {
-  (void)tuple_bytes;
-  const uint64_t bytes = static_cast<uint64_t>(fanout) * ring_lines * kAlignBytes  // rings
-                         + static_cast<uint64_t>(fanout) * 4 * 5                  // per-partition words
-                         + static_cast<uint64_t>(fanout) * ring_lines * 12;       // line queue
-  return bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(bytes);
+  const uint32_t q = tuple_bytes == 8 ? swwc_queue_len<int32_t>(fanout, ring_lines, 1024)
+                                      : swwc_queue_len<int64_t>(fanout, ring_lines, 1024);
+  return swwc_bytes(fanout, ring_lines, q);
 }
 
 hipError_t launch_partition(const PartitionLaunch& a, hipStream_t stream) {
