@@ -19,7 +19,8 @@
 //             aligned 128-byte lines; a tuple that completes a line queues
 //             the line (one queue reservation per wave and step);
 //   flushers: write the lines queued in the previous step, 64/L lines per
-//             store instruction, each a full aligned 128-byte line.
+//             non-temporal store instruction, each a full aligned 128-byte
+//             line.
 // Steps are separated by two barriers (a step's placement and the flush of
 // the previous step's lines, then a commit of the per-partition marks).  A
 // line is read by the flushers one step after it was completed, so the ring
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
       }
 #pragma unroll
       for (uint32_t u = 0; u < UF; ++u)
-        if (live[u] && e[u].y + j >= lo[u]) store_tuple<K>(org(e[u].x) + e[u].y + j, t[u].key, t[u].value);
+        if (live[u] && e[u].y + j >= lo[u]) store_tuple_nt<K>(org(e[u].x) + e[u].y + j, t[u].key, t[u].value);
     }
   };
   // Between steps: the limit of the next step, the low-water mark after a
