@@ -5,13 +5,17 @@
 // launching per join.  A batch joins n pairs with one submission:
 //   * joins whose build side fits one block's LDS table ("small") go to one
 //     persistent launch that joins them without partitioning (batch.hip);
-//   * every other join runs the two-pass radix plan of its size class, with
-//     radix bits chosen by fij_join_autotune; one pooled plan per class,
-//     sized for the largest join seen in that class, is reused by all its
-//     joins;
-//   * the whole submission (the small-join launch and every plan's kernels)
-//     is recorded into one HIP graph and launched once; results are written
-//     to device slots and copied back with one copy.
+//   * every other join runs the two-pass radix pipeline with the radix bits
+//     of its size class (fij_join_autotune); all joins of a class go
+//     through each pass together, one launch per pass: join j's pass-1
+//     partitions are partitions j * F1 .. j * F1 + F1 - 1 of one batched
+//     plan (pipeline.hpp), the histogram, offsets and scatter launches map
+//     their blocks to (join, side, chunk), pass 2 and the join kernels run
+//     over all partitions and add each join's count and sums to its result;
+//     the class's buffers are pooled and grown when a submission needs more;
+//   * the whole submission (the small-join launch and every class's
+//     launches) is recorded into one HIP graph and launched once; results
+//     are written to device slots and copied back with one copy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,6 +31,7 @@
 #include "../../include/fij.h"
 #include "batch.hpp"
 #include "internal.hpp"
+#include "pipeline.hpp"
 
 namespace fij {
 fij_status set_error(fij_status s, const char* msg);
@@ -55,9 +60,31 @@ fij_status batch_fail(fij_status s, const char* fmt, ...) {
 
 uint32_t ceil_log2(uint64_t x) { return x <= 1 ? 0 : 64u - static_cast<uint32_t>(__builtin_clzll(x - 1)); }
 
-struct ClassPlan {
-  fij_join_plan* plan = nullptr;
-  uint64_t build_cap = 0, probe_cap = 0;
+// Buffers of one size class's batched plan and the capacities they were
+// allocated for.
+struct ClassArena {
+  std::vector<void*> allocs;
+  PipelineDevice d{};
+  HistParams* hist = nullptr;      // [2 * joins]
+  BatchScanJob* scan = nullptr;    // [2 * joins]
+  ScatterParams* scat = nullptr;   // [2 * joins]
+  BatchSide* sides = nullptr;      // [2 * joins]
+  uint32_t* prefix = nullptr;      // [2 * joins + 1] first chunk of each join side
+  uint32_t* job_slot = nullptr;    // [joins]
+  uint32_t* counts = nullptr;      // per-chunk histograms of every join side
+  uint64_t* local = nullptr;       // per-chunk write offsets of every join side
+  // capacities
+  uint64_t joins = 0, rel1[2] = {0, 0}, rel2[2] = {0, 0}, tiles[2] = {0, 0}, chunk_words = 0, units = 0;
+  // this submission: joins, chunks of all join sides, pass-2 tiles per side
+  uint32_t sub_joins = 0, sub_blocks = 0, sub_tiles[2] = {0, 0};
+};
+
+// One join side's place in a class arena.
+struct SideLayout {
+  uint64_t n;
+  uint32_t chunks;
+  uint64_t counts_at, local_at, rel1_at, rel2_at;
+  uint32_t tile_at;
 };
 
 }  // namespace
@@ -66,6 +93,7 @@ struct fij_join_batch {
   fij_join_batch_config cfg;
   KeyType kt;
   uint32_t cus;
+  uint32_t lds_total;
   SmallJob* d_jobs = nullptr;
   uint32_t* d_ticket = nullptr;
   JoinAggregate* d_results = nullptr;
@@ -73,8 +101,224 @@ struct fij_join_batch {
   JoinAggregate* h_results = nullptr;
   hipStream_t own = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
-  std::map<std::pair<uint32_t, uint32_t>, ClassPlan> plans;
+  std::map<std::pair<uint32_t, uint32_t>, ClassArena> arenas;
 };
+
+namespace {
+
+void free_arena(ClassArena& a) {
+  for (void* p : a.allocs) (void)hipFree(p);
+  a = ClassArena{};
+}
+
+template <typename T>
+hipError_t arena_alloc(ClassArena& a, T** out, uint64_t count) {
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, std::max<uint64_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return e;
+  a.allocs.push_back(p);
+  *out = static_cast<T*>(p);
+  return hipSuccess;
+}
+
+// Chunks of a batched join side: about 32768 tuples each, at most one per
+// compute unit, and never a chunk length that leaves the last chunk empty.
+uint32_t side_chunks(fij_type type, uint64_t n, uint32_t cus) {
+  uint32_t g = static_cast<uint32_t>(std::min<uint64_t>(cus, std::max<uint64_t>(1, (n + 32767) / 32768)));
+  auto fits = [&](uint32_t chunks) {
+    uint64_t c = 0;
+    return fij_input_chunk_size(type, n, chunks, &c) == FIJ_OK && c * (chunks - 1) <= n;
+  };
+  while (g > 1 && !fits(g)) --g;
+  return g;
+}
+
+// Grows the class arena to hold `joins` joins with these totals; every
+// buffer is allocated anew when any capacity is short (the old contents are
+// not needed between submissions).
+fij_status grow_arena(ClassArena& a, KeyType kt, uint32_t b1, uint32_t b2, uint64_t joins, const uint64_t (&rel1)[2],
+                      const uint64_t (&rel2)[2], const uint64_t (&tiles)[2], uint64_t chunk_words, uint64_t units,
+                      hipStream_t st) {
+  if (a.joins >= joins && a.rel1[0] >= rel1[0] && a.rel1[1] >= rel1[1] && a.rel2[0] >= rel2[0] &&
+      a.rel2[1] >= rel2[1] && a.tiles[0] >= tiles[0] && a.tiles[1] >= tiles[1] && a.chunk_words >= chunk_words &&
+      a.units >= units)
+    return FIJ_OK;
+  BATCH_HIP(hipStreamSynchronize(st));
+  ClassArena n{};
+  // room for growth: a quarter more than this submission needs
+  auto more = [](uint64_t x) { return x + x / 4; };
+  n.joins = more(std::max(a.joins, joins));
+  for (int s = 0; s < 2; ++s) {
+    n.rel1[s] = more(std::max(a.rel1[s], rel1[s]));
+    n.rel2[s] = more(std::max(a.rel2[s], rel2[s]));
+    n.tiles[s] = more(std::max(a.tiles[s], tiles[s]));
+  }
+  n.chunk_words = more(std::max(a.chunk_words, chunk_words));
+  n.units = more(std::max(a.units, units));
+  free_arena(a);
+  const uint32_t F1 = 1u << b1, F2 = 1u << b2;
+  const uint64_t parts = n.joins * F1;
+  const uint64_t tb = tuple_bytes(kt);
+  PipelineDevice& d = n.d;
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < 2 && e == hipSuccess; ++s) {
+    PipelineSide& ps = s ? d.probe : d.build;
+    uint8_t *r1 = nullptr, *r2 = nullptr;
+    if ((e = arena_alloc(n, &ps.off1, parts)) != hipSuccess || (e = arena_alloc(n, &ps.plen, parts)) != hipSuccess ||
+        (e = arena_alloc(n, &ps.pstart, parts + 1)) != hipSuccess ||
+        (e = arena_alloc(n, &ps.tile_start, parts + 1)) != hipSuccess ||
+        (e = arena_alloc(n, &r1, n.rel1[s] * tb)) != hipSuccess || (e = arena_alloc(n, &r2, n.rel2[s] * tb)) != hipSuccess ||
+        (e = arena_alloc(n, &ps.meta, (n.tiles[s] + 1) * F2)) != hipSuccess)
+      break;
+    ps.rel1 = r1;
+    ps.rel2 = r2;
+  }
+  if (e == hipSuccess) e = arena_alloc(n, &n.counts, n.chunk_words);
+  if (e == hipSuccess) e = arena_alloc(n, &n.local, n.chunk_words);
+  if (e == hipSuccess) e = arena_alloc(n, &d.work_counter, kJoinCounterWords);
+  if (e == hipSuccess) e = arena_alloc(n, &d.overflow, parts * F2);
+  if (e == hipSuccess) e = arena_alloc(n, &d.units, n.units);
+  if (e == hipSuccess) e = arena_alloc(n, &d.aggregates, 1);
+  if (e == hipSuccess) e = arena_alloc(n, &n.hist, 2 * n.joins);
+  if (e == hipSuccess) e = arena_alloc(n, &n.scan, 2 * n.joins);
+  if (e == hipSuccess) e = arena_alloc(n, &n.scat, 2 * n.joins);
+  if (e == hipSuccess) e = arena_alloc(n, &n.sides, 2 * n.joins);
+  if (e == hipSuccess) e = arena_alloc(n, &n.prefix, 2 * n.joins + 1);
+  if (e == hipSuccess) e = arena_alloc(n, &n.job_slot, n.joins);
+  // the deferred-item flags start at zero; every join leaves them at zero
+  if (e == hipSuccess) e = hipMemset(d.overflow, 0, parts * F2);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    free_arena(n);
+    return batch_fail(e == hipErrorOutOfMemory ? FIJ_OUT_OF_MEMORY : FIJ_DEVICE_ERROR, "batched plan buffers: %s",
+                      hipGetErrorString(e));
+  }
+  a = n;
+  return FIJ_OK;
+}
+
+// Records one size class's passes over its joins (join indices `ids`).
+fij_status enqueue_class(fij_join_batch* b, ClassArena& a, uint32_t b1, uint32_t b2, const std::vector<uint32_t>& ids,
+                         const fij_join_desc* joins, hipStream_t st, uint32_t lds_total, std::vector<uint8_t>& host) {
+  const KeyType kt = b->kt;
+  const fij_type type = static_cast<fij_type>(b->cfg.type);
+  const uint32_t F1 = 1u << b1, F2 = 1u << b2;
+  const uint32_t pad = pad_tuples(kt);
+  const uint32_t L = 128 / tuple_bytes(kt);
+  const uint32_t tile2 = pipeline_tile2(kt);
+  const uint32_t J = static_cast<uint32_t>(ids.size());
+  std::vector<SideLayout> lay(2 * J);
+  uint64_t rel1[2] = {0, 0}, rel2[2] = {0, 0}, tiles[2] = {0, 0}, words = 0, probe_total = 0;
+  uint32_t blocks = 0;
+  for (uint32_t j = 0; j < J; ++j) {
+    const fij_join_desc& jd = joins[ids[j]];
+    for (int s = 0; s < 2; ++s) {
+      SideLayout& l = lay[2 * j + s];
+      l.n = s ? jd.probe_len : jd.build_len;
+      l.chunks = side_chunks(type, l.n, b->cus);
+      l.counts_at = words;
+      l.local_at = words;
+      words += static_cast<uint64_t>(F1) * l.chunks;
+      l.rel1_at = rel1[s];
+      rel1[s] += (l.n + static_cast<uint64_t>(F1) * pad + L - 1) / L * L;  // regions start on 128-byte lines
+      l.rel2_at = rel2[s];
+      rel2[s] += l.n;
+      l.tile_at = static_cast<uint32_t>(tiles[s]);
+      tiles[s] += (l.n + tile2 - 1) / tile2 + F1;
+      blocks += l.chunks;
+    }
+    probe_total += jd.probe_len;
+  }
+  if (static_cast<uint64_t>(J) * F1 * F2 >= 0xFFFFFFFFull || tiles[0] >= 0xFFFFFFFFull || tiles[1] >= 0xFFFFFFFFull)
+    return batch_fail(FIJ_INTEGER_OVERFLOW, "size class %u+%u: too many joins for one submission", b1, b2);
+  const uint64_t units = static_cast<uint64_t>(J) * F1 * F2 + probe_total / 65536 + J + 1;
+  fij_status s = grow_arena(a, kt, b1, b2, J, rel1, rel2, tiles, words, units, st);
+  if (s != FIJ_OK) return s;
+  // descriptors
+  const size_t n_hist = 2ull * J * sizeof(HistParams), n_scan = 2ull * J * sizeof(BatchScanJob),
+               n_scat = 2ull * J * sizeof(ScatterParams), n_side = 2ull * J * sizeof(BatchSide),
+               n_pre = (2ull * J + 1) * 4, n_slot = static_cast<size_t>(J) * 4;
+  host.assign(n_hist + n_scan + n_scat + n_side + n_pre + n_slot, 0);
+  HistParams* hh = reinterpret_cast<HistParams*>(host.data());
+  BatchScanJob* hs = reinterpret_cast<BatchScanJob*>(host.data() + n_hist);
+  ScatterParams* hc = reinterpret_cast<ScatterParams*>(host.data() + n_hist + n_scan);
+  BatchSide* hb = reinterpret_cast<BatchSide*>(host.data() + n_hist + n_scan + n_scat);
+  uint32_t* hp = reinterpret_cast<uint32_t*>(host.data() + n_hist + n_scan + n_scat + n_side);
+  uint32_t* hj = reinterpret_cast<uint32_t*>(host.data() + n_hist + n_scan + n_scat + n_side + n_pre);
+  const uint64_t tb = tuple_bytes(kt);
+  uint32_t at = 0;
+  for (uint32_t j = 0; j < J; ++j) {
+    const fij_join_desc& jd = joins[ids[j]];
+    hj[j] = ids[j];
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t js = 2 * j + side;
+      const SideLayout& l = lay[js];
+      PipelineSide& ps = side ? a.d.probe : a.d.build;
+      const void* keys = side ? jd.probe_keys : jd.build_keys;
+      const void* pays = side ? jd.probe_payloads : jd.build_payloads;
+      hh[js] = hist_params(kt, keys, l.n, l.chunks, b1, 1024, a.counts + l.counts_at);
+      hs[js] = BatchScanJob{a.counts + l.counts_at, a.local + l.local_at, ps.off1 + static_cast<uint64_t>(j) * F1,
+                            l.rel1_at, F1, l.chunks, pad};
+      void* region = static_cast<uint8_t*>(ps.rel1) + l.rel1_at * tb;
+      const hipError_t e = scatter_params_batched(kt, keys, pays, l.n, l.chunks, b1, a.local + l.local_at, region,
+                                                  lds_total, &hc[js]);
+      if (e != hipSuccess) return batch_fail(FIJ_UNSUPPORTED, "size class %u+%u: pass-1 buffers do not fit LDS", b1, b2);
+      hb[js] = BatchSide{l.rel1_at + l.n + static_cast<uint64_t>(F1) * pad, l.rel2_at, j * F1, l.tile_at};
+      hp[js] = at;
+      at += l.chunks;
+    }
+  }
+  hp[2 * J] = at;
+  BATCH_HIP(hipMemcpyAsync(a.hist, hh, n_hist, hipMemcpyHostToDevice, st));
+  BATCH_HIP(hipMemcpyAsync(a.scan, hs, n_scan, hipMemcpyHostToDevice, st));
+  BATCH_HIP(hipMemcpyAsync(a.scat, hc, n_scat, hipMemcpyHostToDevice, st));
+  BATCH_HIP(hipMemcpyAsync(a.sides, hb, n_side, hipMemcpyHostToDevice, st));
+  BATCH_HIP(hipMemcpyAsync(a.prefix, hp, n_pre, hipMemcpyHostToDevice, st));
+  BATCH_HIP(hipMemcpyAsync(a.job_slot, hj, n_slot, hipMemcpyHostToDevice, st));
+  BATCH_HIP(hipStreamSynchronize(st));
+  // the plan's device view
+  PipelineDevice& d = a.d;
+  d.F1 = J * F1;
+  d.F2 = F2;
+  d.G1 = 1;
+  d.bits1 = b1;
+  d.bits2 = b2;
+  d.pad = pad;
+  d.tile2 = tile2;
+  d.sides = 3;
+  d.agg_len = 0;
+  d.job_results = b->d_results;
+  d.job_slot = a.job_slot;
+  d.job_shift = b1;
+  d.build.n = rel2[0];
+  d.probe.n = rel2[1];
+  a.sub_joins = J;
+  a.sub_blocks = blocks;
+  a.sub_tiles[0] = static_cast<uint32_t>(tiles[0]);
+  a.sub_tiles[1] = static_cast<uint32_t>(tiles[1]);
+  return FIJ_OK;
+}
+
+// The class's launches (after enqueue_class prepared its descriptors): one
+// per pass over all its joins.
+hipError_t launch_class(fij_join_batch* b, ClassArena& a, uint32_t b1, hipStream_t st, uint32_t lds_total) {
+  const KeyType kt = b->kt;
+  const uint32_t js = 2 * a.sub_joins;
+  hipError_t e;
+  if ((e = launch_hist_batched(kt, a.hist, a.prefix, js, a.sub_blocks, b1, 1024, st)) != hipSuccess) return e;
+  if ((e = launch_offsets_batched(a.scan, js, st)) != hipSuccess) return e;
+  if ((e = launch_partition_batched(kt, a.scat, a.prefix, js, a.sub_blocks, b1, lds_total, st)) != hipSuccess) return e;
+  if ((e = launch_pipeline_plan_batched(a.d, a.sides, a.sub_joins, a.d.build.n, a.d.probe.n, a.sub_tiles[0],
+                                        a.sub_tiles[1], st)) != hipSuccess)
+    return e;
+  const uint64_t tiles = static_cast<uint64_t>(a.sub_tiles[0]) + a.sub_tiles[1];
+  const uint32_t sort_grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(b->cus, tiles)));
+  if ((e = launch_pipeline_sort2(a.d, kt, sort_grid, st)) != hipSuccess) return e;
+  if ((e = launch_pipeline_join(a.d, kt, false, false, 2 * b->cus, st)) != hipSuccess) return e;
+  return launch_pipeline_join(a.d, kt, false, true, 2 * b->cus, st);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -112,6 +356,10 @@ fij_status fij_join_batch_create(const fij_join_batch_config* cfg, fij_join_batc
   b->cfg = *cfg;
   b->kt = cfg->type == FIJ_TYPE_I32 ? KeyType::I32 : KeyType::I64;
   b->cus = static_cast<uint32_t>(prop.multiProcessorCount);
+  int optin = 0;
+  b->lds_total = static_cast<uint32_t>(prop.sharedMemPerBlock);
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > 0)
+    b->lds_total = std::max<uint32_t>(b->lds_total, static_cast<uint32_t>(optin));
   const uint64_t n = cfg->max_joins;
   fij_status st = FIJ_OK;
   if (hipMalloc(&b->d_jobs, n * sizeof(SmallJob)) != hipSuccess ||
@@ -134,7 +382,7 @@ fij_status fij_join_batch_create(const fij_join_batch_config* cfg, fij_join_batc
 fij_status fij_join_batch_destroy(fij_join_batch* b) {
   if (!b) return FIJ_OK;
   if (b->own) (void)hipStreamSynchronize(b->own);
-  for (auto& kv : b->plans) fij_join_plan_destroy(kv.second.plan);
+  for (auto& kv : b->arenas) free_arena(kv.second);
   (void)hipFree(b->d_jobs);
   (void)hipFree(b->d_ticket);
   (void)hipFree(b->d_results);
@@ -156,9 +404,8 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
   const fij_type type = static_cast<fij_type>(b->cfg.type);
   const size_t kb = key_bytes(b->kt);
   // classify
-  uint32_t n_small = 0;
-  std::vector<std::pair<std::pair<uint32_t, uint32_t>, uint32_t>> large;  // (class, join index)
-  std::map<std::pair<uint32_t, uint32_t>, std::pair<uint64_t, uint64_t>> need;
+  uint32_t n_small = 0, n_large = 0;
+  std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> classes;  // radix bits -> join indices
   for (uint32_t i = 0; i < n; ++i) {
     const fij_join_desc& j = joins[i];
     for (const void* c : {j.build_keys, j.build_payloads, j.probe_keys, j.probe_payloads})
@@ -179,31 +426,16 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
       sj.probe_len = j.probe_len;
       sj.result = b->d_results + i;
     } else {
-      large.push_back({{b1, b2}, i});
-      auto& m = need[{b1, b2}];
-      m.first = std::max(m.first, j.build_len);
-      m.second = std::max(m.second, j.probe_len);
+      classes[{b1, b2}].push_back(i);
+      ++n_large;
     }
   }
-  // pooled plans: grow a class's plan when a join exceeds it
-  for (auto& kv : need) {
-    ClassPlan& cp = b->plans[kv.first];
-    if (cp.plan && cp.build_cap >= kv.second.first && cp.probe_cap >= kv.second.second) continue;
-    if (cp.plan) {
-      BATCH_HIP(hipStreamSynchronize(st));
-      fij_join_plan_destroy(cp.plan);
-      cp.plan = nullptr;
-    }
-    fij_join_plan_config pc{};
-    pc.type = b->cfg.type;
-    pc.radix_bits_1 = kv.first.first;
-    pc.radix_bits_2 = kv.first.second;
-    pc.build_len = std::max(cp.build_cap, kv.second.first);
-    pc.probe_len = std::max(cp.probe_cap, kv.second.second);
-    fij_status s = fij_join_plan_create(&pc, &cp.plan);
+  // every class's descriptors and (grown) buffers
+  std::vector<uint8_t> host;
+  for (auto& kv : classes) {
+    fij_status s = enqueue_class(b, b->arenas[kv.first], kv.first.first, kv.first.second, kv.second, joins, st,
+                                 b->lds_total, host);
     if (s != FIJ_OK) return s;
-    cp.build_cap = pc.build_len;
-    cp.probe_cap = pc.probe_len;
   }
   if (n_small) {
     BATCH_HIP(hipMemcpyAsync(b->d_jobs, b->h_jobs, n_small * sizeof(SmallJob), hipMemcpyHostToDevice, st));
@@ -220,11 +452,11 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
     hipError_t e = launch_small_joins(b->kt, b->d_jobs, n_small, b->d_ticket, b->cus, st);
     if (e != hipSuccess) s = batch_fail(FIJ_DEVICE_ERROR, "small joins: %s", hipGetErrorString(e));
   }
-  for (size_t k = 0; k < large.size() && s == FIJ_OK; ++k) {
-    const fij_join_desc& j = joins[large[k].second];
-    s = fij_join_plan_enqueue_sized(b->plans[large[k].first].plan, j.build_keys, j.build_payloads, j.build_len,
-                                    j.probe_keys, j.probe_payloads, j.probe_len,
-                                    reinterpret_cast<fij_join_aggregate*>(b->d_results + large[k].second), st);
+  for (auto& kv : classes) {
+    if (s != FIJ_OK) break;
+    hipError_t e = launch_class(b, b->arenas[kv.first], kv.first.first, st, b->lds_total);
+    if (e != hipSuccess) s = batch_fail(FIJ_DEVICE_ERROR, "size class %u+%u: %s", kv.first.first, kv.first.second,
+                                        hipGetErrorString(e));
   }
   if (use_graph) {
     hipError_t e = hipStreamEndCapture(st, &graph);
@@ -262,8 +494,8 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
     stats->prepare_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
     stats->total_ms = std::chrono::duration<float, std::milli>(t2 - t0).count();
     stats->small_joins = n_small;
-    stats->partitioned_joins = static_cast<uint32_t>(large.size());
-    stats->size_classes = static_cast<uint32_t>(need.size());
+    stats->partitioned_joins = n_large;
+    stats->size_classes = static_cast<uint32_t>(classes.size());
     stats->graph = use_graph ? 1u : 0u;
   }
   if (exec) (void)hipGraphExecDestroy(exec);
@@ -274,9 +506,14 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
 fij_status fij_join_batch_device_bytes(const fij_join_batch* b, uint64_t* bytes) {
   if (!b || !bytes) return batch_fail(FIJ_INVALID_ARGUMENT, "NULL argument");
   uint64_t total = static_cast<uint64_t>(b->cfg.max_joins) * (sizeof(SmallJob) + sizeof(JoinAggregate)) + 256;
-  for (const auto& kv : b->plans) {
-    uint64_t x = 0;
-    if (fij_join_plan_device_bytes(kv.second.plan, &x) == FIJ_OK) total += x;
+  const uint64_t tb = tuple_bytes(b->kt);
+  for (const auto& kv : b->arenas) {
+    const ClassArena& a = kv.second;
+    const uint64_t parts = a.joins << kv.first.first, F2 = 1ull << kv.first.second;
+    for (int s = 0; s < 2; ++s)
+      total += (a.rel1[s] + a.rel2[s]) * tb + parts * (8 + 4 + 8 + 4) + (a.tiles[s] + 1) * F2 * 4;
+    total += a.chunk_words * (4 + 8) + parts * F2 + a.units * 16 +
+             a.joins * 2 * (sizeof(HistParams) + sizeof(BatchScanJob) + sizeof(ScatterParams) + sizeof(BatchSide) + 4);
   }
   *bytes = total;
   return FIJ_OK;

@@ -226,6 +226,30 @@ inline uint32_t dynamic_lds_budget(const void* kernel, uint32_t total) {
 
 // ------------------------------------------------------------- memory ----
 
+// Global-memory pointers.  A pointer read from memory (a descriptor of a
+// batched launch) is generic, and accesses through it would be flat
+// instructions, which count on both vmcnt and lgkmcnt; as_global makes
+// them global loads and stores.
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ global_ptr<T> as_global(T* p) {
+  return (global_ptr<T>)(p);
+}
+
+// Wave-uniform load of a word or a struct (a multiple of 4 bytes) through
+// the scalar data cache, for data no kernel in flight writes.
+template <typename U>
+__device__ __forceinline__ U scalar_load(const U* ptr) {
+  static_assert(sizeof(U) % 4 == 0, "scalar loads move whole words");
+  U out;
+  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+  const __attribute__((address_space(4))) uint32_t* in = (const __attribute__((address_space(4))) uint32_t*)(ptr);
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(U) / 4; ++i) o[i] = in[i];
+  return out;
+}
+
 template <typename K>
 struct Vec2;
 template <>
@@ -251,6 +275,20 @@ __device__ __forceinline__ void store_tuple_nt(Tuple<K>* dst, K key, K value) {
   v.x = key;
   v.y = value;
   __builtin_nontemporal_store(v, reinterpret_cast<typename Vec2<K>::type*>(dst));
+}
+template <typename K>
+__device__ __forceinline__ void store_tuple(global_ptr<Tuple<K>> dst, K key, K value) {
+  typename Vec2<K>::type v;
+  v.x = key;
+  v.y = value;
+  *(global_ptr<typename Vec2<K>::type>)(dst) = v;
+}
+template <typename K>
+__device__ __forceinline__ void store_tuple_nt(global_ptr<Tuple<K>> dst, K key, K value) {
+  typename Vec2<K>::type v;
+  v.x = key;
+  v.y = value;
+  __builtin_nontemporal_store(v, (global_ptr<typename Vec2<K>::type>)(dst));
 }
 template <typename K>
 __device__ __forceinline__ Tuple<K> load_tuple(const Tuple<K>* src) {

@@ -16,22 +16,6 @@ constexpr uint32_t kHistBlockMax = 1024;
 constexpr uint32_t kHistLdsBudget = 48 * 1024;
 constexpr uint32_t kWaveTmpWords = 32;  // u64 scratch for block scans
 
-struct HistParams {
-  const void* keys;
-  const void* pays;
-  void* dst_keys;
-  void* dst_pays;
-  uint64_t n;
-  uint64_t chunk_len;
-  uint32_t radix_bits;
-  uint32_t ignore_bits;
-  uint32_t pad;
-  uint32_t subhists;
-  uint64_t* offsets;   // chunked: [G][F]
-  uint32_t* counts;    // contiguous: [F][G]
-  uint64_t* scan_state;
-  uint64_t scan_state_len;
-};
 
 template <typename K>
 struct VecN {
@@ -42,8 +26,8 @@ struct VecN {
 // Shared tail of both histogram kernels: sub-histogram reduction and output.
 template <bool CHUNKED>
 __device__ void finish_histogram(uint32_t* hist, uint64_t* wave_tmp, uint32_t F, uint32_t S,
-                                 const HistParams& p) {
-  const uint32_t tid = threadIdx.x, c = blockIdx.x, G = gridDim.x;
+                                 const HistParams& p, uint32_t c, uint32_t G) {
+  const uint32_t tid = threadIdx.x;
   __syncthreads();
   for (uint32_t i = tid; i < F; i += blockDim.x) {
     uint32_t s = 0;
@@ -63,15 +47,16 @@ __device__ void finish_histogram(uint32_t* hist, uint64_t* wave_tmp, uint32_t F,
   }
 }
 
+// Histogram of chunk c of G (one block).
 template <typename K, bool CHUNKED, bool COPY>
-__global__ __launch_bounds__(kHistBlockMax) void hist_kernel(HistParams p) {
+__device__ __forceinline__ void hist_body(const HistParams& p, uint32_t c, uint32_t G) {
   extern __shared__ uint64_t lds_u64[];
   uint64_t* wave_tmp = lds_u64;
   uint32_t* hist = reinterpret_cast<uint32_t*>(lds_u64 + kWaveTmpWords);
   const uint32_t F = 1u << p.radix_bits;
   const uint32_t S = p.subhists;
   const uint64_t mask = radix_mask(p.radix_bits, p.ignore_bits);
-  const uint32_t tid = threadIdx.x, c = blockIdx.x, G = gridDim.x, B = blockDim.x;
+  const uint32_t tid = threadIdx.x, B = blockDim.x;
 
   if (p.scan_state) {
     for (uint64_t i = static_cast<uint64_t>(c) * B + tid; i < p.scan_state_len;
@@ -102,7 +87,7 @@ __global__ __launch_bounds__(kHistBlockMax) void hist_kernel(HistParams p) {
     using VT = typename V::type;
     auto load = [&](VT (&x)[U], uint64_t v0) {
 #pragma unroll
-      for (uint32_t u = 0; u < U; ++u) x[u] = __builtin_nontemporal_load(kv + min(v0 + u * B + tid, nv - 1));
+      for (uint32_t u = 0; u < U; ++u) x[u] = __builtin_nontemporal_load(as_global(kv) + min(v0 + u * B + tid, nv - 1));
     };
     auto count = [&](const VT (&x)[U], uint64_t v0) {
 #pragma unroll
@@ -150,7 +135,64 @@ __global__ __launch_bounds__(kHistBlockMax) void hist_kernel(HistParams p) {
     }
     wave_count_add(my, key_to_partition<K>(key, mask, p.ignore_bits), act);
   }
-  finish_histogram<CHUNKED>(hist, wave_tmp, F, S, p);
+  finish_histogram<CHUNKED>(hist, wave_tmp, F, S, p, c, G);
+}
+
+template <typename K, bool CHUNKED, bool COPY>
+__global__ __launch_bounds__(kHistBlockMax) void hist_kernel(HistParams p) {
+  hist_body<K, CHUNKED, COPY>(p, blockIdx.x, gridDim.x);
+}
+
+// Index of the job that owns `block`: the last j with prefix[j] <= block
+// (prefix[0] = 0, prefix[jobs] = total blocks).
+__device__ __forceinline__ uint32_t job_of_block(const uint32_t* prefix, uint32_t jobs, uint32_t block) {
+  uint32_t lo = 0, hi = jobs;
+  while (hi - lo > 1) {
+    const uint32_t m = (lo + hi) / 2;
+    if (scalar_load(prefix + m) <= block) lo = m; else hi = m;
+  }
+  return lo;
+}
+
+// Histograms of many relations in one launch: block b counts chunk
+// b - prefix[j] of relation j (batched joins).
+template <typename K>
+__global__ __launch_bounds__(kHistBlockMax) void hist_batched_kernel(const HistParams* jobs, const uint32_t* prefix,
+                                                                    uint32_t njobs) {
+  const uint32_t j = job_of_block(prefix, njobs, blockIdx.x);
+  const HistParams p = scalar_load(jobs + j);
+  const uint32_t c0 = scalar_load(prefix + j);
+  hist_body<K, false, false>(p, blockIdx.x - c0, scalar_load(prefix + j + 1) - c0);
+}
+
+// Offsets of many relations in one launch (one block per relation): the
+// exclusive scan of counts[F][G] in partition-major order gives
+// local[c][p] = scan + (p + 1) * pad (relative to the relation's region) and
+// off1[p] = region + local[0][p].
+__global__ __launch_bounds__(1024) void offsets_batched_kernel(const BatchScanJob* jobs) {
+  __shared__ uint64_t wave_tmp[kWaveTmpWords];
+  const BatchScanJob j = jobs[blockIdx.x];
+  const uint32_t tid = threadIdx.x, B = blockDim.x;
+  const uint64_t n = static_cast<uint64_t>(j.fanout) * j.chunks;
+  const uint64_t per = (n + B - 1) / B;
+  const uint64_t lo = min(tid * per, n), hi = min(lo + per, n);
+  uint64_t local = 0;
+  for (uint64_t e = lo; e < hi; ++e) local += j.counts[e];
+  const uint64_t incl = wave_inclusive_scan(local);
+  const uint32_t w = tid / kWave;
+  if (lane_id() == kWave - 1) wave_tmp[w] = incl;
+  __syncthreads();
+  uint64_t before = 0;
+  for (uint32_t i = 0; i < w; ++i) before += wave_tmp[i];
+  uint64_t run = before + incl - local;
+  for (uint64_t e = lo; e < hi; ++e) {
+    const uint32_t part = static_cast<uint32_t>(e / j.chunks);
+    const uint32_t chunk = static_cast<uint32_t>(e - static_cast<uint64_t>(part) * j.chunks);
+    const uint64_t off = run + static_cast<uint64_t>(part + 1) * j.pad;
+    j.local[static_cast<uint64_t>(chunk) * j.fanout + part] = off;
+    if (chunk == 0) j.off1[part] = j.region + off;
+    run += j.counts[e];
+  }
 }
 
 struct TransformParams {
@@ -245,7 +287,7 @@ __global__ __launch_bounds__(kHistBlockMax) void transform_counts_kernel(Transfo
   }
   HistParams hp{};
   hp.counts = p.counts;
-  finish_histogram<false>(hist, wave_tmp, F, S, hp);
+  finish_histogram<false>(hist, wave_tmp, F, S, hp, c, G);
 }
 
 // Decoupled look-back (Merrill & Garland, "Single-pass Parallel Prefix Scan
@@ -439,6 +481,38 @@ hipError_t launch_transform_counts(KeyType t, const void* src_rel, const uint64_
     hipLaunchKernelGGL(transform_counts_kernel<int32_t>, dim3(chunks), dim3(block), lds, stream, p);
   else
     hipLaunchKernelGGL(transform_counts_kernel<int64_t>, dim3(chunks), dim3(block), lds, stream, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_hist_batched(KeyType t, const HistParams* jobs, const uint32_t* prefix, uint32_t njobs,
+                               uint32_t blocks, uint32_t radix_bits, uint32_t block, hipStream_t stream) {
+  if (blocks == 0) return hipSuccess;
+  const uint32_t F = 1u << radix_bits;
+  const size_t lds = kWaveTmpWords * 8 + static_cast<size_t>(choose_subhists(F, block)) * F * 4;
+  if (t == KeyType::I32)
+    hipLaunchKernelGGL(hist_batched_kernel<int32_t>, dim3(blocks), dim3(block), lds, stream, jobs, prefix, njobs);
+  else
+    hipLaunchKernelGGL(hist_batched_kernel<int64_t>, dim3(blocks), dim3(block), lds, stream, jobs, prefix, njobs);
+  return hipGetLastError();
+}
+
+HistParams hist_params(KeyType t, const void* keys, uint64_t n, uint32_t chunks, uint32_t radix_bits,
+                       uint32_t block, uint32_t* counts) {
+  HistParams p{};
+  p.keys = keys;
+  p.n = n;
+  p.chunk_len = t == KeyType::I32 ? canonical_chunk_len<int32_t>(n, chunks) : canonical_chunk_len<int64_t>(n, chunks);
+  p.radix_bits = radix_bits;
+  p.ignore_bits = 0;
+  p.pad = pad_tuples(t);
+  p.subhists = choose_subhists(1u << radix_bits, block);
+  p.counts = counts;
+  return p;
+}
+
+hipError_t launch_offsets_batched(const BatchScanJob* jobs, uint32_t njobs, hipStream_t stream) {
+  if (njobs == 0) return hipSuccess;
+  hipLaunchKernelGGL(offsets_batched_kernel, dim3(njobs), dim3(1024), 0, stream, jobs);
   return hipGetLastError();
 }
 

@@ -56,6 +56,45 @@ hipError_t launch_offsets_scan(const uint32_t* counts, uint32_t fanout, uint32_t
                                uint32_t pad, uint64_t* local_offsets, uint64_t* offsets,
                                uint64_t* scan_state, hipStream_t stream);
 
+// Histogram launch parameters (one relation, contiguous counts[F][chunks] or
+// chunked offsets[chunks][F]).
+struct HistParams {
+  const void* keys;
+  const void* pays;
+  void* dst_keys;
+  void* dst_pays;
+  uint64_t n;
+  uint64_t chunk_len;
+  uint32_t radix_bits;
+  uint32_t ignore_bits;
+  uint32_t pad;
+  uint32_t subhists;
+  uint64_t* offsets;   // chunked: [G][F]
+  uint32_t* counts;    // contiguous: [F][G]
+  uint64_t* scan_state;
+  uint64_t scan_state_len;
+};
+
+// One relation of a batched offsets launch: counts[F][chunks] in, write
+// offsets local[chunks][F] (relative to the relation's region) and
+// partition starts off1[F] (in the plan's pass-1 array) out.
+struct BatchScanJob {
+  const uint32_t* counts;
+  uint64_t* local;
+  uint64_t* off1;
+  uint64_t region;
+  uint32_t fanout, chunks, pad;
+};
+// Contiguous histogram parameters of one relation of a batched launch.
+HistParams hist_params(KeyType t, const void* keys, uint64_t n, uint32_t chunks, uint32_t radix_bits,
+                       uint32_t block, uint32_t* counts);
+// Histograms of njobs relations in one launch: block b counts chunk
+// b - prefix[j] of jobs[j] (prefix[njobs] = blocks, device memory).
+hipError_t launch_hist_batched(KeyType t, const HistParams* jobs, const uint32_t* prefix, uint32_t njobs,
+                               uint32_t blocks, uint32_t radix_bits, uint32_t block, hipStream_t stream);
+// Offsets of njobs relations in one launch (one block each).
+hipError_t launch_offsets_batched(const BatchScanJob* jobs, uint32_t njobs, hipStream_t stream);
+
 // ---- partitioning (partition.hip) -----------------------------------------
 struct PartitionLaunch {
   KeyType type;
@@ -79,6 +118,29 @@ struct PartitionLaunch {
   uint32_t spill_from = 0;
 };
 hipError_t launch_partition(const PartitionLaunch& a, hipStream_t stream);
+// Buffered-scatter parameters of one relation of a batched launch.
+struct ScatterParams {
+  const void* keys;
+  const void* pays;
+  uint64_t n;
+  uint64_t chunk_len;
+  uint32_t radix_bits;
+  uint32_t ignore_bits;
+  const uint64_t* chunk_offsets;
+  void* relation;
+  uint32_t ring_lines;
+  uint32_t queue_len;    // line queue entries per step buffer
+  uintptr_t spill_addr;  // address tuple index 0 would have in the spill region
+  uint32_t spill_from;   // first partition written to the spill region
+};
+
+hipError_t scatter_params_batched(KeyType t, const void* keys, const void* pays, uint64_t n, uint32_t chunks,
+                                  uint32_t radix_bits, const uint64_t* chunk_offsets, void* relation,
+                                  uint32_t lds_total, ScatterParams* out);
+// Scatters of njobs relations of one fanout in one launch: block b scatters
+// chunk b - prefix[j] of jobs[j] (prefix[njobs] = blocks, device memory).
+hipError_t launch_partition_batched(KeyType t, const ScatterParams* jobs, const uint32_t* prefix, uint32_t njobs,
+                                    uint32_t blocks, uint32_t radix_bits, uint32_t lds_total, hipStream_t stream);
 // Pass 1 on the CPU (host memory in and out): the contiguous layout of the
 // GPU's pass 1; offsets[F] receives the partition starts.
 void cpu_partition_contiguous(KeyType t, const void* keys, const void* pays, uint64_t n, uint32_t bits,

@@ -35,20 +35,6 @@
 namespace fij {
 namespace {
 
-struct ScatterParams {
-  const void* keys;
-  const void* pays;
-  uint64_t n;
-  uint64_t chunk_len;
-  uint32_t radix_bits;
-  uint32_t ignore_bits;
-  const uint64_t* chunk_offsets;
-  void* relation;
-  uint32_t ring_lines;
-  uint32_t queue_len;    // line queue entries per step buffer
-  uintptr_t spill_addr;  // address tuple index 0 would have in the spill region
-  uint32_t spill_from;   // first partition written to the spill region
-};
 
 // Destination of partition d's tuples: the relation or, from spill_from on,
 // the spill region (both indexed by the relation's tuple index).
@@ -61,9 +47,10 @@ __device__ __forceinline__ T* part_dst(const ScatterParams& p, T* rel, uint64_t 
 }
 
 template <typename K>
-__device__ __forceinline__ void chunk_range(const ScatterParams& p, uint64_t& begin, uint64_t& len) {
-  begin = min(p.chunk_len * blockIdx.x, p.n);
-  len = (blockIdx.x + 1 == gridDim.x) ? p.n - begin : min(p.chunk_len, p.n - begin);
+__device__ __forceinline__ void chunk_range(const ScatterParams& p, uint32_t chunk, uint32_t G, uint64_t& begin,
+                                            uint64_t& len) {
+  begin = min(p.chunk_len * chunk, p.n);
+  len = (chunk + 1 == G) ? p.n - begin : min(p.chunk_len, p.n - begin);
 }
 
 // Direct scatter (reference NC): per-chunk LDS cursors, wave-aggregated ranks.
@@ -78,7 +65,7 @@ __global__ __launch_bounds__(1024) void scatter_direct_kernel(ScatterParams p) {
   for (uint32_t i = tid; i < F; i += B) cursors[i] = static_cast<uint32_t>(off[i] - cbase);
   __syncthreads();
   uint64_t begin, len;
-  chunk_range<K>(p, begin, len);
+  chunk_range<K>(p, blockIdx.x, gridDim.x, begin, len);
   const K* keys = static_cast<const K*>(p.keys) + begin;
   const K* pays = static_cast<const K*>(p.pays) + begin;
   Tuple<K>* rel = static_cast<Tuple<K>*>(p.relation) + cbase;
@@ -100,13 +87,16 @@ __global__ __launch_bounds__(1024) void scatter_direct_kernel(ScatterParams p) {
 constexpr uint32_t kScatterPrefetch = 8;
 
 // V keys (and V payloads) per loader lane and step: one 16-byte load each.
+// The type is only as aligned as its elements: a column is aligned to its
+// element size, and gfx950 loads 16 bytes from any 4-byte aligned address.
 template <typename K>
 struct Vec16 {
-  typedef K type __attribute__((ext_vector_type(16 / sizeof(K))));
+  typedef K type __attribute__((ext_vector_type(16 / sizeof(K)), aligned(sizeof(K))));
 };
 
+// Buffered scatter of chunk `chunk` of G (one block).
 template <typename K, int BLOCK, bool SPILL>
-__global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
+__device__ __forceinline__ void scatter_swwc_body(const ScatterParams& p, uint32_t chunk, uint32_t G) {
   using T = Tuple<K>;
   using VK = typename Vec16<K>::type;
   constexpr uint32_t V = 16 / sizeof(K);          // tuples per loader lane and step
@@ -140,7 +130,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   uint32_t* ring_lo = fprev + F;       // positions below are never written from the ring
   uint32_t* ring_lo_prev = ring_lo + F;  // ring_lo during the previous step
 
-  const uint64_t* off = p.chunk_offsets + static_cast<uint64_t>(blockIdx.x) * F;
+  const uint64_t* off = p.chunk_offsets + static_cast<uint64_t>(chunk) * F;
   const uint64_t cbase = off[0];
   T* const rel = static_cast<T*>(p.relation) + cbase;
   const uint32_t a = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(rel) % kAlignBytes) / sizeof(T));
@@ -162,7 +152,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
   __syncthreads();
 
   uint64_t begin, len;
-  chunk_range<K>(p, begin, len);
+  chunk_range<K>(p, chunk, G, begin, len);
   const K* keys = static_cast<const K*>(p.keys) + begin;
   const K* pays = static_cast<const K*>(p.pays) + begin;
   const uint64_t nvec = len / V;
@@ -196,7 +186,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
           ring[static_cast<size_t>(d[k]) * RL + (pos[k] & (RL - 1))] = t;
           complete = (pos[k] & (L - 1)) == L - 1;
         } else {
-          store_tuple<K>(org(d[k]) + pos[k], key[k], pay[k]);
+          store_tuple<K>(as_global(org(d[k]) + pos[k]), key[k], pay[k]);
         }
       }
       cm[k] = ballot(complete);
@@ -235,7 +225,8 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
       }
 #pragma unroll
       for (uint32_t u = 0; u < UF; ++u)
-        if (live[u] && e[u].y + j >= lo[u]) store_tuple_nt<K>(org(e[u].x) + e[u].y + j, t[u].key, t[u].value);
+        if (live[u] && e[u].y + j >= lo[u])
+          store_tuple_nt<K>(as_global(org(e[u].x) + e[u].y + j), t[u].key, t[u].value);
     }
   };
   // Between steps: the limit of the next step, the low-water mark after a
@@ -267,11 +258,13 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
       // past the last full vector re-read it, so every step issues the
       // same loads.
       VK kv[D], pv[D];
+      auto load = [&](VK& kb, VK& pb, uint64_t v) {
+        kb = __builtin_nontemporal_load(as_global(reinterpret_cast<const VK*>(keys)) + v);
+        pb = __builtin_nontemporal_load(as_global(reinterpret_cast<const VK*>(pays)) + v);
+      };
 #pragma unroll
       for (uint32_t b = 0; b < D; ++b) {
-        const uint64_t v = min(static_cast<uint64_t>(b) * NLT + lt, nvec - 1);
-        kv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(keys) + v);
-        pv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(pays) + v);
+        load(kv[b], pv[b], min(static_cast<uint64_t>(b) * NLT + lt, nvec - 1));
         // issued in buffer order: the loop's waits count on it
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -284,9 +277,7 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
           // registers stay in place across the loop's back edge (a copy of a
           // register with a load in flight would wait for that load).
           __builtin_amdgcn_sched_barrier(0);
-          const uint64_t v = min((s + D) * NLT + lt, nvec - 1);
-          kv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(keys) + v);
-          pv[b] = __builtin_nontemporal_load(reinterpret_cast<const VK*>(pays) + v);
+          load(kv[b], pv[b], min((s + D) * NLT + lt, nvec - 1));
           __syncthreads();
           commit(static_cast<uint32_t>(s & 1) ^ 1u);
           __syncthreads();
@@ -333,9 +324,32 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
     const uint32_t pos = (f & ~(L - 1)) + (idx % L);
     if (pos < f && pos >= ring_lo[i]) {
       const T t = ring[static_cast<size_t>(i) * RL + (pos & (RL - 1))];
-      store_tuple<K>(org(i) + pos, t.key, t.value);
+      store_tuple<K>(as_global(org(i) + pos), t.key, t.value);
     }
   }
+}
+
+template <typename K, int BLOCK, bool SPILL>
+__global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
+  scatter_swwc_body<K, BLOCK, SPILL>(p, blockIdx.x, gridDim.x);
+}
+
+// Many relations in one launch: block b scatters chunk b - prefix[j] of
+// relation j (batched joins; every relation has the same fanout).
+template <typename K>
+__global__ __launch_bounds__(1024) void scatter_swwc_batched_kernel(const ScatterParams* __restrict__ jobs,
+                                                                    const uint32_t* __restrict__ prefix,
+                                                                    uint32_t njobs) {
+  uint32_t lo = 0, hi = njobs;  // the last j with prefix[j] <= blockIdx.x
+  while (hi - lo > 1) {
+    const uint32_t m = (lo + hi) / 2;
+    if (scalar_load(prefix + m) <= blockIdx.x) lo = m; else hi = m;
+  }
+  // (read-only, unaliased descriptors: the compiler treats the pointers
+  // loaded from them as global)
+  const ScatterParams p = jobs[lo];
+  const uint32_t c0 = scalar_load(prefix + lo);
+  scatter_swwc_body<K, 1024, false>(p, blockIdx.x - c0, scalar_load(prefix + lo + 1) - c0);
 }
 
 // Queue entries per step buffer: a step completes at most one line per
@@ -395,10 +409,7 @@ hipError_t launch_partition_t(const PartitionLaunch& a, hipStream_t stream) {
   const bool spill = a.spill != nullptr && a.spill_from < F;
   p.spill_from = spill ? a.spill_from : F;
   p.spill_addr = reinterpret_cast<uintptr_t>(a.spill) - static_cast<uintptr_t>(a.spill_base * sizeof(Tuple<K>));
-  // The buffered scatter loads 16 bytes per lane: both columns must be
-  // 16-byte aligned (chunk starts are multiples of 128 bytes from them).
-  const bool vec_ok = reinterpret_cast<uintptr_t>(a.keys) % 16 == 0 && reinterpret_cast<uintptr_t>(a.pays) % 16 == 0;
-  if (a.buffered && vec_ok) {
+  if (a.buffered) {
     hipError_t e;
     if (spill)
       e = launch_swwc_t<K, 1024, true>(p, F, a.chunks, a.lds_bytes, stream);
@@ -430,6 +441,63 @@ uint32_t swwc_lds_bytes(uint32_t fanout, uint32_t tuple_bytes, uint32_t ring_lin
 hipError_t launch_partition(const PartitionLaunch& a, hipStream_t stream) {
   if (a.type == KeyType::I32) return launch_partition_t<int32_t>(a, stream);
   return launch_partition_t<int64_t>(a, stream);
+}
+
+namespace {
+
+template <typename K>
+const void* batched_scatter_kernel() {
+  return reinterpret_cast<const void*>(scatter_swwc_batched_kernel<K>);
+}
+
+// Deepest ring of the batched scatter for this fanout (0: none fits).
+template <typename K>
+uint32_t batched_ring(uint32_t fanout, uint32_t lds_total) {
+  const uint32_t budget = dynamic_lds_budget(batched_scatter_kernel<K>(), lds_total);
+  for (uint32_t r = 256; r >= 1; r /= 2)
+    if (swwc_bytes(fanout, r, swwc_queue_len<K>(fanout, r, 1024)) <= budget) return r;
+  return 0;
+}
+
+}  // namespace
+
+hipError_t scatter_params_batched(KeyType t, const void* keys, const void* pays, uint64_t n, uint32_t chunks,
+                                  uint32_t radix_bits, const uint64_t* chunk_offsets, void* relation,
+                                  uint32_t lds_total, ScatterParams* out) {
+  const uint32_t F = 1u << radix_bits;
+  const uint32_t ring = t == KeyType::I32 ? batched_ring<int32_t>(F, lds_total) : batched_ring<int64_t>(F, lds_total);
+  if (ring == 0) return hipErrorNotSupported;
+  ScatterParams p{};
+  p.keys = keys;
+  p.pays = pays;
+  p.n = n;
+  p.chunk_len = t == KeyType::I32 ? canonical_chunk_len<int32_t>(n, chunks) : canonical_chunk_len<int64_t>(n, chunks);
+  p.radix_bits = radix_bits;
+  p.ignore_bits = 0;
+  p.chunk_offsets = chunk_offsets;
+  p.relation = relation;
+  p.ring_lines = ring;
+  p.queue_len = t == KeyType::I32 ? swwc_queue_len<int32_t>(F, ring, 1024) : swwc_queue_len<int64_t>(F, ring, 1024);
+  p.spill_from = F;
+  *out = p;
+  return hipSuccess;
+}
+
+hipError_t launch_partition_batched(KeyType t, const ScatterParams* jobs, const uint32_t* prefix, uint32_t njobs,
+                                    uint32_t blocks, uint32_t radix_bits, uint32_t lds_total, hipStream_t stream) {
+  if (blocks == 0) return hipSuccess;
+  const uint32_t F = 1u << radix_bits;
+  const uint32_t ring = t == KeyType::I32 ? batched_ring<int32_t>(F, lds_total) : batched_ring<int64_t>(F, lds_total);
+  if (ring == 0) return hipErrorNotSupported;
+  const uint32_t ql = t == KeyType::I32 ? swwc_queue_len<int32_t>(F, ring, 1024) : swwc_queue_len<int64_t>(F, ring, 1024);
+  const uint32_t lds = swwc_bytes(F, ring, ql);
+  const void* kern = t == KeyType::I32 ? batched_scatter_kernel<int32_t>() : batched_scatter_kernel<int64_t>();
+  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (t == KeyType::I32)
+    hipLaunchKernelGGL(scatter_swwc_batched_kernel<int32_t>, dim3(blocks), dim3(1024), lds, stream, jobs, prefix, njobs);
+  else
+    hipLaunchKernelGGL(scatter_swwc_batched_kernel<int64_t>, dim3(blocks), dim3(1024), lds, stream, jobs, prefix, njobs);
+  return hipGetLastError();
 }
 
 }  // namespace fij

@@ -75,6 +75,54 @@ __global__ __launch_bounds__(1024) void plan_kernel(PipelineDevice d) {
   }
 }
 
+// Batched plans: one block per join side (sides[2 * join + side]) computes
+// the lengths, pass-2 starts and tile map of that side's F1j partitions
+// (partitions part_base .. part_base + F1j - 1 of the plan, contiguous layout
+// up to rel1_end); block 0 also writes the totals and resets the join
+// counters, each join's side-0 block clears its result.
+__global__ __launch_bounds__(1024) void plan_batched_kernel(PipelineDevice d, const BatchSide* sides, uint32_t F1j,
+                                                            uint64_t rel2_total_b, uint64_t rel2_total_p,
+                                                            uint32_t tiles_total_b, uint32_t tiles_total_p) {
+  extern __shared__ uint64_t buf[];
+  __shared__ uint64_t wave_tmp[32];
+  const uint32_t js = blockIdx.x, side = js & 1u;
+  const PipelineSide& s = side ? d.probe : d.build;
+  const BatchSide bs = sides[js];
+  uint64_t* starts = buf;       // [F1j]
+  uint64_t* tiles = buf + F1j;  // [F1j]
+  for (uint32_t p = threadIdx.x; p < F1j; p += blockDim.x) {
+    const uint32_t vp = bs.part_base + p;
+    const uint64_t b = s.off1[vp];
+    const uint64_t e = p + 1 < F1j ? s.off1[vp + 1] - d.pad : bs.rel1_end;
+    const uint32_t n = static_cast<uint32_t>(e - b);
+    s.plen[vp] = n;
+    starts[p] = n;
+    tiles[p] = d.bits2 ? (n + d.tile2 - 1) / d.tile2 : 0;
+  }
+  __syncthreads();
+  block_exclusive_scan<uint64_t>(starts, F1j, wave_tmp);
+  block_exclusive_scan<uint64_t>(tiles, F1j, wave_tmp);
+  for (uint32_t p = threadIdx.x; p < F1j; p += blockDim.x) {
+    s.pstart[bs.part_base + p] = bs.rel2_base + starts[p];
+    s.tile_start[bs.part_base + p] = bs.tile_base + static_cast<uint32_t>(tiles[p]);
+  }
+  if (js == 0) {
+    if (threadIdx.x == 0) {
+      d.build.pstart[d.F1] = rel2_total_b;
+      d.probe.pstart[d.F1] = rel2_total_p;
+      d.build.tile_start[d.F1] = tiles_total_b;
+      d.probe.tile_start[d.F1] = tiles_total_p;
+    }
+    if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
+  }
+  if (side == 0 && threadIdx.x == 0) {
+    JoinAggregate* r = d.job_results + d.job_slot[js >> 1];
+    r->count = 0;
+    r->sum_build = 0;
+    r->sum_probe = 0;
+  }
+}
+
 // Wave-uniform read of data that no launch in flight writes, through the
 // scalar data cache: it is counted apart from the vector loads, so waiting
 // for it does not wait for loads already in flight.
@@ -91,6 +139,7 @@ template <typename K>
 struct TileRef {
   uint32_t rel, p, j, n;
   const Tuple<K>* src;
+  bool skip;  // a tile index between two joins' tile ranges of a batched plan
 };
 
 // Pass 2 of both relations (the sides in d.sides) as one persistent launch:
@@ -127,8 +176,11 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     }
     r.p = lo;
     r.j = u - uniform_load(s.tile_start + lo);
-    r.n = min(TILE2, uniform_load(s.plen + lo) - r.j * TILE2);
-    r.src = static_cast<const T*>(s.rel1) + uniform_load(s.off1 + lo) + static_cast<uint64_t>(r.j) * TILE2;
+    const uint32_t plen = uniform_load(s.plen + lo);
+    r.skip = static_cast<uint64_t>(r.j) * TILE2 >= plen;
+    // a skipped tile loads one in-bounds tuple and is not processed
+    r.n = r.skip ? 1u : min(TILE2, plen - r.j * TILE2);
+    r.src = static_cast<const T*>(s.rel1) + (r.skip ? 0 : uniform_load(s.off1 + lo) + static_cast<uint64_t>(r.j) * TILE2);
     return r;
   };
   // lanes past the tile's end re-read its last tuple: unconditional loads
@@ -137,6 +189,7 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     for (uint32_t k = 0; k < ITEMS; ++k) v[k] = load_tuple_nt<K>(r.src + min(k * BLOCK + tid, r.n - 1));
   };
   auto process = [&](const TileRef<K>& r, const T (&v)[ITEMS]) {
+    if (r.skip) return;  // uniform
     const PipelineSide& s = r.rel ? d.probe : d.build;
     uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -195,8 +248,9 @@ constexpr uint32_t kMaxRuns = 64;
 
 // Runs of one side of item (p, q): with two passes, one per pass-2 tile of
 // pass-1 partition p; with one pass, partition p cut into tile2-tuple pieces.
+// (The pass-2 tiles of p are ceil(plen / tile2) from tile_start[p] on; in a
+// batched plan the joins' tile ranges have gaps between them.)
 __device__ __forceinline__ uint32_t run_count(const PipelineDevice& d, const PipelineSide& s, uint32_t p) {
-  if (d.bits2) return uniform_load(s.tile_start + p + 1) - uniform_load(s.tile_start + p);
   return (uniform_load(s.plen + p) + d.tile2 - 1) / d.tile2;
 }
 
@@ -515,6 +569,24 @@ __device__ __forceinline__ void join_block_aggregate(const PipelineDevice& d, ui
   }
 }
 
+// Batched plans: the wave's {count, SUM(build.value), SUM(probe.value)}
+// added to the result of join `job`, then cleared.  Called by whole waves.
+__device__ __forceinline__ void wave_job_aggregate(const PipelineDevice& d, uint32_t job, uint64_t& cnt, uint64_t& sb,
+                                                   uint64_t& sp) {
+  const uint64_t wc = wave_broadcast_last(wave_inclusive_scan(cnt));
+  const uint64_t wb = wave_broadcast_last(wave_inclusive_scan(sb));
+  const uint64_t wp = wave_broadcast_last(wave_inclusive_scan(sp));
+  if (lane_id() == 0 && (wc | wb | wp)) {
+    JoinAggregate* r = d.job_results + d.job_slot[job];
+    atomicAdd(reinterpret_cast<unsigned long long*>(&r->count), static_cast<unsigned long long>(wc));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&r->sum_build), static_cast<unsigned long long>(wb));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&r->sum_probe), static_cast<unsigned long long>(wp));
+  }
+  cnt = 0;
+  sb = 0;
+  sp = 0;
+}
+
 // The per-wave LDS layout shared by both join kernels: the table (keys, then
 // values) and the two sides' run batches.
 template <typename K, uint32_t CAP>
@@ -561,7 +633,7 @@ __device__ __forceinline__ void lane_item_side(const PipelineDevice& d, const Pi
     return;
   }
   const uint32_t t0 = s.tile_start[p];
-  runs = live ? s.tile_start[p + 1] - t0 : 0u;
+  runs = live ? (s.plen[p] + d.tile2 - 1) / d.tile2 : 0u;
   uint32_t most = runs;
 #pragma unroll
   for (uint32_t off = 32; off > 0; off >>= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), off)));
@@ -647,10 +719,11 @@ __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice
         join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, __builtin_amdgcn_readfirstlane(unit.y),
                                                   __builtin_amdgcn_readfirstlane(unit.z), l.table_keys, l.rb, l.rp,
                                                   rel_b, rel_p, bt, pt, cnt, sb, sp);
+        if (d.job_results) wave_job_aggregate(d, (item / F2) >> d.job_shift, cnt, sb, sp);
       }
     }
   }
-  join_block_aggregate<WAVES>(d, cnt, sb, sp);
+  if (!d.job_results) join_block_aggregate<WAVES>(d, cnt, sb, sp);
 }
 
 // Block-wide batch of up to BLOCK runs of one side of an item, one run per
@@ -762,10 +835,11 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
         }
         __syncthreads();  // probes done; the batch (and then the table) may be overwritten
       }
+      if (d.job_results) wave_job_aggregate(d, p >> d.job_shift, cnt, sb, sp);
       clear();
     }
   }
-  join_block_aggregate<BLOCK / 64>(d, cnt, sb, sp);
+  if (!d.job_results) join_block_aggregate<BLOCK / 64>(d, cnt, sb, sp);
 }
 
 // One wave per work item (p, q).  Items are handed out in tickets of up to 8
@@ -851,12 +925,20 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
   if (cur < items) fetch(cur);
   bool pend = false;  // an item's tuples are in flight in the work buffer
   uint32_t pend_tb = 0, pend_tp = 0;
+  // batched plans: the join whose result cnt / sb / sp belong to, and that
+  // of the pending item
+  const bool batched = d.job_results != nullptr;
+  uint32_t acc_job = 0, pend_job = 0;
   // One pipeline step: stage item `cur` into (sbuf, spbuf) and finish the
   // pending item held in (wbuf, wpbuf).  Returns false when no items are
   // left (the pending item, if any, is finished first).  The build/probe has
   // a single call site, which keeps the register allocation of the two
   // buffers simple.
   auto step = [&](T (&sbuf)[BPL], T (&spbuf)[SPL], T (&wbuf)[BPL], T (&wpbuf)[SPL]) -> bool {
+    if (batched && pend && pend_job != acc_job) {
+      wave_job_aggregate(d, acc_job, cnt, sb, sp);
+      acc_job = pend_job;
+    }
     const bool live = cur < items;
     bool staged = false;
     uint32_t tb = 0, tp = 0, nxt = items;
@@ -904,6 +986,7 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
     pend = staged;
     pend_tb = tb;
     pend_tp = tp;
+    if (batched && staged) pend_job = (cur / F2) >> d.job_shift;
     cur = nxt;
     return live;
   };
@@ -912,7 +995,10 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
     if (!step(ab, ap, bb, bp)) break;
     if (!step(bb, bp, ab, ap)) break;
   }
-  join_block_aggregate<WAVES>(d, cnt, sb, sp);
+  if (batched)
+    wave_job_aggregate(d, acc_job, cnt, sb, sp);
+  else
+    join_block_aggregate<WAVES>(d, cnt, sb, sp);
 }
 
 template <typename K, uint32_t CAP, uint32_t WAVES>
@@ -966,6 +1052,16 @@ uint32_t pipeline_tile2(KeyType t) { (void)t; return 8192u; }
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream) {
   const uint32_t blocks = d.sides == 3 ? 2u : 1u;
   hipLaunchKernelGGL(plan_kernel, dim3(blocks), dim3(1024), 2 * d.F1 * 8, stream, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_pipeline_plan_batched(const PipelineDevice& d, const BatchSide* sides, uint32_t joins,
+                                        uint64_t rel2_total_b, uint64_t rel2_total_p, uint32_t tiles_total_b,
+                                        uint32_t tiles_total_p, hipStream_t stream) {
+  if (joins == 0) return hipSuccess;
+  const uint32_t F1j = d.F1 / joins;
+  hipLaunchKernelGGL(plan_batched_kernel, dim3(2 * joins), dim3(1024), 2 * F1j * 8, stream, d, sides, F1j,
+                     rel2_total_b, rel2_total_p, tiles_total_b, tiles_total_p);
   return hipGetLastError();
 }
 

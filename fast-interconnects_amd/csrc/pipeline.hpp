@@ -61,6 +61,22 @@ struct PipelineDevice {
   uint4* units;               // [unit capacity] {item, first probe run, end probe run, 1 = wave table}
   JoinAggregate* aggregates;  // [agg_len]
   uint32_t agg_len;
+  // Many joins in one plan (batched joins): pass-1 partition p belongs to
+  // join p >> job_shift, whose count and sums are added to
+  // job_results[job_slot[join]] instead of the per-block aggregates.
+  // job_results == nullptr: one join.
+  JoinAggregate* job_results;
+  const uint32_t* job_slot;
+  uint32_t job_shift;
+};
+
+// One join side of a batched plan: its pass-1 region and where its
+// partitions start in the batched plan's arrays (plan_batched_kernel).
+struct BatchSide {
+  uint64_t rel1_end;    // one past the side's pass-1 region in rel1 (tuples)
+  uint64_t rel2_base;   // first tuple of the side's pass-2 output in rel2
+  uint32_t part_base;   // first pass-1 partition (join * F1)
+  uint32_t tile_base;   // first pass-2 tile (the joins before it hold at most their capacity)
 };
 
 // Allocates one side's buffers (capacity n tuples, F1 x F2 partitions, G1
@@ -77,6 +93,12 @@ hipError_t enqueue_side_scatter(const PipelineDevice& d, PipelineSide& s, KeyTyp
 uint32_t pipeline_tile2(KeyType t);
 size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
+// Partition lengths, pass-2 starts and tile maps of every join side of a
+// batched plan (sides[2 * join + side]); resets the join counters and the
+// joins' results.  pstart / tile_start [F1] hold the totals.
+hipError_t launch_pipeline_plan_batched(const PipelineDevice& d, const BatchSide* sides, uint32_t joins,
+                                        uint64_t rel2_total_b, uint64_t rel2_total_p, uint32_t tiles_total_b,
+                                        uint32_t tiles_total_p, hipStream_t stream);
 // Pass 2 of both relations in one persistent launch of `grid` blocks.
 hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t grid, hipStream_t stream);
 // dst = the sum of the n per-block aggregates src (one block).
