@@ -67,6 +67,67 @@ def test_batch_null_keys_and_duplicates():
 
 
 
+def views_at_odd_offsets(host_joins, dtype):
+    """All columns in one device buffer, each starting 1-3 elements past the
+    previous column's end (aligned to the element size only)."""
+    cols = [a for j in host_joins for a in j]
+    gaps = [1 + i % 3 for i in range(len(cols))]
+    total = sum(len(c) for c in cols) + sum(gaps)
+    buf = np.zeros(total, dtype)
+    at, starts = 0, []
+    for c, g in zip(cols, gaps):
+        at += g
+        buf[at:at + len(c)] = c
+        starts.append(at)
+        at += len(c)
+    dev = to_dev(buf)
+    views = [dev.view(s_, s_ + len(c)) for s_, c in zip(starts, cols)]
+    return [tuple(views[4 * i:4 * i + 4]) for i in range(len(host_joins))]
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_batch_unaligned_columns_and_growth(dtype):
+    """Partitioned joins of several size classes, columns at odd element
+    offsets; a second, larger submission grows the size classes' pooled
+    buffers, a third, smaller one reuses them."""
+    rng = np.random.default_rng(11)
+    small = [make_join(i, int(2 ** rng.uniform(14, 16)), int(rng.integers(4, 9)), dtype, seed=300) for i in range(6)]
+    large = [make_join(i, int(2 ** rng.uniform(14, 19)), int(rng.integers(4, 17)), dtype, seed=400) for i in range(20)]
+    batch = fij.JoinBatch(dtype, 20)
+    for host in (small, large, small[:3]):
+        got, stats = batch.execute(views_at_odd_offsets(host, dtype))
+        assert stats["partitioned_joins"] == len(host)
+        for j, (h, g) in enumerate(zip(host, got)):
+            assert g == orc.sort_merge_join(*h), f"join {j} ({len(h[0])} x {len(h[2])})"
+    assert stats["size_classes"] >= 1
+    assert fij.device_status() == 0
+
+
+def test_batch_deferred_items():
+    """Heavy build and probe keys inside batched partitioned joins: their
+    work items exceed the register join and are joined by the deferred
+    kernels, whose counts and sums go to each join's own result."""
+    dtype = np.int32
+    host = []
+    # (n, copies of the heavy key on the build side, on the probe side):
+    # a probe-heavy item with a short build side (block-table units), one
+    # with more build tuples than a block table holds (wave-table units), a
+    # heavy probe key against a single build tuple; the last two share a
+    # size class, so one batched plan routes two joins' deferred items
+    for i, (n, hb, hp) in enumerate([(30000, 800, 40000), (60000, 7000, 5000), (90000, 1, 100000),
+                                     (100000, 2000, 20000)]):
+        rk = orc.gen_perm(n, 70 + i, dtype, 1)
+        rk[:hb] = 7 + i
+        rp = orc.gen_iota(n, 1, dtype)
+        sk = orc.gen_uniform(8 * n, 70 + i, 2, 1, n, dtype)
+        sk[:hp] = 7 + i
+        sp = orc.gen_iota(8 * n, 0, dtype)
+        host.append((rk, rp, sk, sp))
+    got, stats = fij.JoinBatch(dtype, len(host)).execute([tuple(to_dev(a) for a in j) for j in host])
+    assert stats["partitioned_joins"] == len(host)
+    assert got == [orc.sort_merge_join(*h) for h in host]
+
+
 def test_plan_enqueue_sized_matches_execute():
     """fij_join_plan_enqueue_sized: joins shorter than the plan, result summed
     into a device address (the form the batch records into its graph)."""

@@ -28,7 +28,16 @@ def make_input(n, dtype, seed, key_kind="pk"):
     return keys, pays
 
 
-def run_partition(n, dtype, hist, algo, bits, grid, block=128, seed=1, key_kind="pk", ignore=0):
+def unaligned_dev(a, shift):
+    """A device view of `a` that starts `shift` elements into its buffer
+    (aligned to the element size only)."""
+    if shift == 0:
+        return to_dev(a)
+    buf = to_dev(np.concatenate([np.zeros(shift, a.dtype), a]))
+    return buf.view(shift, shift + len(a))
+
+
+def run_partition(n, dtype, hist, algo, bits, grid, block=128, seed=1, key_kind="pk", ignore=0, shift=(0, 0)):
     keys, pays = make_input(n, dtype, seed, key_kind)
     kt = fij.KeyType.I32 if dtype == np.int32 else fij.KeyType.I64
     rb = fij.RadixBits(bits) if ignore == 0 else fij.RadixBits(ignore, bits)
@@ -36,7 +45,7 @@ def run_partition(n, dtype, hist, algo, bits, grid, block=128, seed=1, key_kind=
     part = fij.GpuRadixPartitioner(hist, algo, rb, grid, block)
     offs = fij.PartitionOffsets(hist, grid, bits, kt)
     rel = fij.PartitionedRelation(n, hist, bits, grid, kt, fill_null=True)
-    dk, dp = to_dev(keys), to_dev(pays)
+    dk, dp = unaligned_dev(keys, shift[0]), unaligned_dev(pays, shift[1])
     part.prefix_sum(rpass, dk, offs)
     part.partition(rpass, dk, dp, offs, rel)
     sync()
@@ -103,14 +112,28 @@ def test_partition_wide_grid(dtype, bits):
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
-@pytest.mark.parametrize("tiles,extra", [(0, 1), (0, -1), (1, 0), (1, 1), (2, 0), (2, 7), (3, 0), (3, 5),
-                                         (4, -1), (5, 3), (6, 0)])
-def test_partition_tile_counts(dtype, tiles, extra):
-    """The buffered scatter works in tiles of 1024 x 4 (i32) / 1024 x 2 (i64)
-    tuples, three tiles per loop turn: one chunk of every remainder."""
-    tile = 1024 * (4 if dtype == np.int32 else 2)
-    n = tiles * tile + (extra if extra >= 0 else tile + extra)
+@pytest.mark.parametrize("steps,extra", [(0, 1), (0, "v-1"), (1, 0), (1, "v+1"), (7, 3), (8, 0), (8, 1), (9, -1),
+                                         (16, 5), (17, 0)])
+def test_partition_tile_counts(dtype, steps, extra):
+    """The buffered scatter's loaders take 512 x V tuples per step (V = 4
+    for i32, 2 for i64: one 16-byte load per lane and column), eight steps
+    per loop round, and the last len % V tuples apart: one chunk of every
+    kind of remainder."""
+    v = 16 // np.dtype(dtype).itemsize
+    step = 512 * v
+    extra = {"v-1": v - 1, "v+1": v + 1}.get(extra, extra)
+    n = steps * step + (extra if extra >= 0 else step + extra)
     run_partition(n, dtype, CHUNKED, fij.RadixPartitionAlgorithm.SSWWCv2, 9, grid=1, block=1024)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("hist", [CHUNKED, CONTIG])
+@pytest.mark.parametrize("shift", [(1, 3), (2, 0), (0, 1)])
+def test_partition_unaligned_columns(dtype, hist, shift):
+    """Columns aligned to their element size only (views starting 1-3
+    elements into a buffer): the buffered scatter loads 16 bytes per lane
+    from any element-aligned address."""
+    run_partition(300_007, dtype, hist, fij.RadixPartitionAlgorithm.SSWWCv2, 9, grid=37, block=1024, shift=shift)
 
 
 @pytest.mark.parametrize("hist", [CHUNKED, CONTIG])
