@@ -354,10 +354,13 @@ fij_status fij_join_plan_execute(fij_join_plan *plan, const void *build_keys, co
 /* Many independent joins with one submission (no counterpart in the
  * reference, which joins one pair per run of the radix-join binary).  Joins
  * whose build side fits one block's LDS table are joined together by one
- * persistent launch without partitioning; every other join runs the radix
- * plan of its size class (fij_join_autotune) from a pooled plan per class;
- * the submission is recorded into one HIP graph when use_graph is set.
- * Results are per join, identical to joining each pair alone. */
+ * persistent launch without partitioning; every other join is partitioned
+ * in two passes with the radix bits of its size class (fij_join_autotune),
+ * all joins of a class together: one launch per pass (histograms, offsets,
+ * scatter, pass 2, join) covers every join of the class.  The submission is
+ * recorded into one HIP graph when use_graph is set.  Results are per join,
+ * identical to joining each pair alone.  Columns need only be aligned to
+ * their element size. */
 
 typedef struct fij_join_desc {
   const void *build_keys;
@@ -393,9 +396,9 @@ fij_status fij_join_autotune(fij_type type, uint64_t build_len, uint64_t probe_l
                              uint32_t *radix_bits_1, uint32_t *radix_bits_2);
 fij_status fij_join_batch_create(const fij_join_batch_config *config, fij_join_batch **out);
 fij_status fij_join_batch_destroy(fij_join_batch *batch);
-/* Joins joins[0..n) (device columns) and writes results[0..n) (host).  The
- * first call for a size class larger than any before allocates that class's
- * plan; later calls reuse it.  Synchronous. */
+/* Joins joins[0..n) (device columns) and writes results[0..n) (host).  A
+ * size class's buffers are kept between calls and grown (by a quarter more
+ * than asked) when a call needs more.  Synchronous. */
 fij_status fij_join_batch_execute(fij_join_batch *batch, uint32_t n, const fij_join_desc *joins,
                                   fij_join_aggregate *results, fij_join_batch_stats *stats, void *stream);
 fij_status fij_join_batch_device_bytes(const fij_join_batch *batch, uint64_t *bytes);
