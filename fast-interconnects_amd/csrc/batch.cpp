@@ -71,6 +71,7 @@ struct ClassArena {
   BatchSide* sides = nullptr;      // [2 * joins]
   uint32_t* prefix = nullptr;      // [2 * joins + 1] first chunk of each join side
   uint32_t* job_slot = nullptr;    // [joins]
+  uint32_t* side_tiles = nullptr;  // [2 * joins] plan scratch: tiles of each join side
   uint32_t* counts = nullptr;      // per-chunk histograms of every join side
   uint64_t* local = nullptr;       // per-chunk write offsets of every join side
   // capacities
@@ -84,7 +85,6 @@ struct SideLayout {
   uint64_t n;
   uint32_t chunks;
   uint64_t counts_at, local_at, rel1_at, rel2_at;
-  uint32_t tile_at;
 };
 
 }  // namespace
@@ -185,6 +185,7 @@ fij_status grow_arena(ClassArena& a, KeyType kt, uint32_t b1, uint32_t b2, uint6
   if (e == hipSuccess) e = arena_alloc(n, &n.sides, 2 * n.joins);
   if (e == hipSuccess) e = arena_alloc(n, &n.prefix, 2 * n.joins + 1);
   if (e == hipSuccess) e = arena_alloc(n, &n.job_slot, n.joins);
+  if (e == hipSuccess) e = arena_alloc(n, &n.side_tiles, 2 * n.joins);
   // the deferred-item flags start at zero; every join leaves them at zero
   if (e == hipSuccess) e = hipMemset(d.overflow, 0, parts * F2);
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -223,8 +224,7 @@ fij_status enqueue_class(fij_join_batch* b, ClassArena& a, uint32_t b1, uint32_t
       rel1[s] += (l.n + static_cast<uint64_t>(F1) * pad + L - 1) / L * L;  // regions start on 128-byte lines
       l.rel2_at = rel2[s];
       rel2[s] += l.n;
-      l.tile_at = static_cast<uint32_t>(tiles[s]);
-      tiles[s] += (l.n + tile2 - 1) / tile2 + F1;
+      tiles[s] += (l.n + tile2 - 1) / tile2 + F1;  // at most
       blocks += l.chunks;
     }
     probe_total += jd.probe_len;
@@ -263,7 +263,7 @@ fij_status enqueue_class(fij_join_batch* b, ClassArena& a, uint32_t b1, uint32_t
       const hipError_t e = scatter_params_batched(kt, keys, pays, l.n, l.chunks, b1, a.local + l.local_at, region,
                                                   lds_total, &hc[js]);
       if (e != hipSuccess) return batch_fail(FIJ_UNSUPPORTED, "size class %u+%u: pass-1 buffers do not fit LDS", b1, b2);
-      hb[js] = BatchSide{l.rel1_at + l.n + static_cast<uint64_t>(F1) * pad, l.rel2_at, j * F1, l.tile_at};
+      hb[js] = BatchSide{l.rel1_at + l.n + static_cast<uint64_t>(F1) * pad, l.rel2_at, j * F1, 0};
       hp[js] = at;
       at += l.chunks;
     }
@@ -308,8 +308,8 @@ hipError_t launch_class(fij_join_batch* b, ClassArena& a, uint32_t b1, hipStream
   if ((e = launch_hist_batched(kt, a.hist, a.prefix, js, a.sub_blocks, b1, 1024, st)) != hipSuccess) return e;
   if ((e = launch_offsets_batched(a.scan, js, st)) != hipSuccess) return e;
   if ((e = launch_partition_batched(kt, a.scat, a.prefix, js, a.sub_blocks, b1, lds_total, st)) != hipSuccess) return e;
-  if ((e = launch_pipeline_plan_batched(a.d, a.sides, a.sub_joins, a.d.build.n, a.d.probe.n, a.sub_tiles[0],
-                                        a.sub_tiles[1], st)) != hipSuccess)
+  if ((e = launch_pipeline_plan_batched(a.d, a.sides, a.sub_joins, a.side_tiles, a.d.build.n, a.d.probe.n, st)) !=
+      hipSuccess)
     return e;
   const uint64_t tiles = static_cast<uint64_t>(a.sub_tiles[0]) + a.sub_tiles[1];
   const uint32_t sort_grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(b->cus, tiles)));

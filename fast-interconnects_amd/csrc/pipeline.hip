@@ -78,11 +78,14 @@ __global__ __launch_bounds__(1024) void plan_kernel(PipelineDevice d) {
 // Batched plans: one block per join side (sides[2 * join + side]) computes
 // the lengths, pass-2 starts and tile map of that side's F1j partitions
 // (partitions part_base .. part_base + F1j - 1 of the plan, contiguous layout
-// up to rel1_end); block 0 also writes the totals and resets the join
-// counters, each join's side-0 block clears its result.
+// up to rel1_end); the tile map is relative to the side's first tile, whose
+// count goes to side_tiles[join side] (tile_bases_kernel and
+// tile_offset_kernel then make the tiles of all joins one range).  Block 0
+// also writes the pass-2 totals and resets the join counters, each join's
+// side-0 block clears its result.
 __global__ __launch_bounds__(1024) void plan_batched_kernel(PipelineDevice d, const BatchSide* sides, uint32_t F1j,
-                                                            uint64_t rel2_total_b, uint64_t rel2_total_p,
-                                                            uint32_t tiles_total_b, uint32_t tiles_total_p) {
+                                                            uint32_t* side_tiles, uint64_t rel2_total_b,
+                                                            uint64_t rel2_total_p) {
   extern __shared__ uint64_t buf[];
   __shared__ uint64_t wave_tmp[32];
   const uint32_t js = blockIdx.x, side = js & 1u;
@@ -101,17 +104,16 @@ __global__ __launch_bounds__(1024) void plan_batched_kernel(PipelineDevice d, co
   }
   __syncthreads();
   block_exclusive_scan<uint64_t>(starts, F1j, wave_tmp);
-  block_exclusive_scan<uint64_t>(tiles, F1j, wave_tmp);
+  const uint64_t total_tiles = block_exclusive_scan<uint64_t>(tiles, F1j, wave_tmp);
   for (uint32_t p = threadIdx.x; p < F1j; p += blockDim.x) {
     s.pstart[bs.part_base + p] = bs.rel2_base + starts[p];
-    s.tile_start[bs.part_base + p] = bs.tile_base + static_cast<uint32_t>(tiles[p]);
+    s.tile_start[bs.part_base + p] = static_cast<uint32_t>(tiles[p]);
   }
+  if (threadIdx.x == 0) side_tiles[js] = static_cast<uint32_t>(total_tiles);
   if (js == 0) {
     if (threadIdx.x == 0) {
       d.build.pstart[d.F1] = rel2_total_b;
       d.probe.pstart[d.F1] = rel2_total_p;
-      d.build.tile_start[d.F1] = tiles_total_b;
-      d.probe.tile_start[d.F1] = tiles_total_p;
     }
     if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
   }
@@ -121,6 +123,45 @@ __global__ __launch_bounds__(1024) void plan_batched_kernel(PipelineDevice d, co
     r->sum_build = 0;
     r->sum_probe = 0;
   }
+}
+
+// One block: side_tiles[2 * join + side] becomes the exclusive scan, over
+// the joins, of that side's tile counts (in place), and tile_start[F1] of
+// each side the total.
+__global__ __launch_bounds__(1024) void tile_bases_kernel(PipelineDevice d, uint32_t* side_tiles, uint32_t joins) {
+  __shared__ uint32_t wave_tmp[16];
+  const uint32_t tid = threadIdx.x, w = tid / kWave;
+  for (uint32_t side = 0; side < 2; ++side) {
+    uint32_t carry = 0;
+    for (uint32_t j0 = 0; j0 < joins; j0 += 1024) {
+      const uint32_t j = j0 + tid;
+      const uint32_t x = j < joins ? side_tiles[2 * j + side] : 0u;
+      const uint32_t incl = wave_inclusive_scan(x);
+      if (lane_id() == kWave - 1) wave_tmp[w] = incl;
+      __syncthreads();
+      uint32_t before = 0, total = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < 16; ++i) {
+        before += i < w ? wave_tmp[i] : 0u;
+        total += wave_tmp[i];
+      }
+      if (j < joins) side_tiles[2 * j + side] = carry + before + incl - x;
+      carry += total;
+      __syncthreads();
+    }
+    if (tid == 0) (side ? d.probe : d.build).tile_start[d.F1] = carry;
+  }
+}
+
+// tile_start[p] += the first tile of p's join side (all partitions of both
+// sides; F1j partitions per join).
+__global__ __launch_bounds__(256) void tile_offset_kernel(PipelineDevice d, const uint32_t* side_tiles, uint32_t F1j) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * d.F1) return;
+  const uint32_t side = i >= d.F1 ? 1u : 0u;
+  const uint32_t p = i - side * d.F1;
+  uint32_t* ts = side ? d.probe.tile_start : d.build.tile_start;
+  ts[p] += side_tiles[2 * (p / F1j) + side];
 }
 
 // Wave-uniform read of data that no launch in flight writes, through the
@@ -139,7 +180,6 @@ template <typename K>
 struct TileRef {
   uint32_t rel, p, j, n;
   const Tuple<K>* src;
-  bool skip;  // a tile index between two joins' tile ranges of a batched plan
 };
 
 // Pass 2 of both relations (the sides in d.sides) as one persistent launch:
@@ -176,11 +216,8 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     }
     r.p = lo;
     r.j = u - uniform_load(s.tile_start + lo);
-    const uint32_t plen = uniform_load(s.plen + lo);
-    r.skip = static_cast<uint64_t>(r.j) * TILE2 >= plen;
-    // a skipped tile loads one in-bounds tuple and is not processed
-    r.n = r.skip ? 1u : min(TILE2, plen - r.j * TILE2);
-    r.src = static_cast<const T*>(s.rel1) + (r.skip ? 0 : uniform_load(s.off1 + lo) + static_cast<uint64_t>(r.j) * TILE2);
+    r.n = min(TILE2, uniform_load(s.plen + lo) - r.j * TILE2);
+    r.src = static_cast<const T*>(s.rel1) + uniform_load(s.off1 + lo) + static_cast<uint64_t>(r.j) * TILE2;
     return r;
   };
   // lanes past the tile's end re-read its last tuple: unconditional loads
@@ -189,7 +226,6 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     for (uint32_t k = 0; k < ITEMS; ++k) v[k] = load_tuple_nt<K>(r.src + min(k * BLOCK + tid, r.n - 1));
   };
   auto process = [&](const TileRef<K>& r, const T (&v)[ITEMS]) {
-    if (r.skip) return;  // uniform
     const PipelineSide& s = r.rel ? d.probe : d.build;
     uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -248,9 +284,8 @@ constexpr uint32_t kMaxRuns = 64;
 
 // Runs of one side of item (p, q): with two passes, one per pass-2 tile of
 // pass-1 partition p; with one pass, partition p cut into tile2-tuple pieces.
-// (The pass-2 tiles of p are ceil(plen / tile2) from tile_start[p] on; in a
-// batched plan the joins' tile ranges have gaps between them.)
 __device__ __forceinline__ uint32_t run_count(const PipelineDevice& d, const PipelineSide& s, uint32_t p) {
+  if (d.bits2) return uniform_load(s.tile_start + p + 1) - uniform_load(s.tile_start + p);
   return (uniform_load(s.plen + p) + d.tile2 - 1) / d.tile2;
 }
 
@@ -633,7 +668,7 @@ __device__ __forceinline__ void lane_item_side(const PipelineDevice& d, const Pi
     return;
   }
   const uint32_t t0 = s.tile_start[p];
-  runs = live ? (s.plen[p] + d.tile2 - 1) / d.tile2 : 0u;
+  runs = live ? s.tile_start[p + 1] - t0 : 0u;
   uint32_t most = runs;
 #pragma unroll
   for (uint32_t off = 32; off > 0; off >>= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), off)));
@@ -1056,12 +1091,18 @@ hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream) {
 }
 
 hipError_t launch_pipeline_plan_batched(const PipelineDevice& d, const BatchSide* sides, uint32_t joins,
-                                        uint64_t rel2_total_b, uint64_t rel2_total_p, uint32_t tiles_total_b,
-                                        uint32_t tiles_total_p, hipStream_t stream) {
+                                        uint32_t* side_tiles, uint64_t rel2_total_b, uint64_t rel2_total_p,
+                                        hipStream_t stream) {
   if (joins == 0) return hipSuccess;
   const uint32_t F1j = d.F1 / joins;
   hipLaunchKernelGGL(plan_batched_kernel, dim3(2 * joins), dim3(1024), 2 * F1j * 8, stream, d, sides, F1j,
-                     rel2_total_b, rel2_total_p, tiles_total_b, tiles_total_p);
+                     side_tiles, rel2_total_b, rel2_total_p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(tile_bases_kernel, dim3(1), dim3(1024), 0, stream, d, side_tiles, joins);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(tile_offset_kernel, dim3((2 * d.F1 + 255) / 256), dim3(256), 0, stream, d,
+                     static_cast<const uint32_t*>(side_tiles), F1j);
   return hipGetLastError();
 }
 

@@ -76,7 +76,7 @@ struct BatchSide {
   uint64_t rel1_end;    // one past the side's pass-1 region in rel1 (tuples)
   uint64_t rel2_base;   // first tuple of the side's pass-2 output in rel2
   uint32_t part_base;   // first pass-1 partition (join * F1)
-  uint32_t tile_base;   // first pass-2 tile (the joins before it hold at most their capacity)
+  uint32_t reserved;
 };
 
 // Allocates one side's buffers (capacity n tuples, F1 x F2 partitions, G1
@@ -94,11 +94,13 @@ uint32_t pipeline_tile2(KeyType t);
 size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
 // Partition lengths, pass-2 starts and tile maps of every join side of a
-// batched plan (sides[2 * join + side]); resets the join counters and the
-// joins' results.  pstart / tile_start [F1] hold the totals.
+// batched plan (sides[2 * join + side]; the tiles of all joins of a side
+// form one range, side_tiles[2 * joins] is scratch); resets the join
+// counters and the joins' results.  pstart / tile_start [F1] hold the
+// totals.
 hipError_t launch_pipeline_plan_batched(const PipelineDevice& d, const BatchSide* sides, uint32_t joins,
-                                        uint64_t rel2_total_b, uint64_t rel2_total_p, uint32_t tiles_total_b,
-                                        uint32_t tiles_total_p, hipStream_t stream);
+                                        uint32_t* side_tiles, uint64_t rel2_total_b, uint64_t rel2_total_p,
+                                        hipStream_t stream);
 // Pass 2 of both relations in one persistent launch of `grid` blocks.
 hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t grid, hipStream_t stream);
 // dst = the sum of the n per-block aggregates src (one block).
