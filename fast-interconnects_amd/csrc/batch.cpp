@@ -330,12 +330,14 @@ fij_status fij_join_autotune(fij_type type, uint64_t build_len, uint64_t probe_l
   // one block joins a small pair: its build side fits one LDS table fill and
   // its probe side is short enough for one block to stream
   *small = build_len <= small_join_table_tuples(kt) && probe_len <= (1u << 22) ? 1u : 0u;
-  // Size classes of the radix plan: at most 512 build tuples per (pass-1,
-  // pass-2) partition pair on average, so that pairs fit the join's
-  // registers (at most 640), split evenly over the passes (pass 1 at most 9
-  // bits: its LDS line buffers), and enough pass-1 partitions that a pair's
-  // probe side has at most 64 pass-2 tiles of 8192 tuples.
-  uint32_t b = std::max<uint32_t>(2, ceil_log2((build_len + 511) / 512));
+  // Size classes of the radix plan: at most 512 build and 512 probe tuples
+  // per (pass-1, pass-2) partition pair on average, so that pairs fit the
+  // join's registers (at most 640 per side), split evenly over the passes
+  // (pass 1 at most 9 bits: its LDS line buffers), and enough pass-1
+  // partitions that a pair's probe side has at most 64 pass-2 tiles of 8192
+  // tuples.
+  uint32_t b = std::max<uint32_t>(
+      2, std::max(ceil_log2((build_len + 511) / 512), ceil_log2((probe_len + 511) / 512)));
   uint32_t b1 = std::min<uint32_t>(9, (b + 1) / 2);
   const uint32_t need = ceil_log2(probe_len) > 19 ? ceil_log2(probe_len) - 19 : 0;
   b1 = std::min<uint32_t>(9, std::max(b1, need));

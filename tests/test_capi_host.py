@@ -297,8 +297,9 @@ def test_autotune_rule():
         for f in (1, 4, 16):
             small, b1, b2 = fij.autotune(np.int32, n_r, n_r * f)
             assert 1 <= b1 <= 9 and 1 <= b2 <= 12
-            # at most 512 build tuples per final partition on average
+            # at most 512 build and 512 probe tuples per final partition on average
             assert n_r >> (b1 + b2) <= 512 or (b1, b2) == (9, 12)
+            assert (n_r * f) >> (b1 + b2) <= 512 or (b1, b2) == (9, 12)
             # a pass-1 partition's probe side spans at most 64 pass-2 tiles of 8192
             assert (n_r * f) >> b1 <= 64 * 8192 or b1 == 9
 
