@@ -652,19 +652,26 @@ constexpr uint64_t kUnitTuples = 65536;
 // and the counter word kOverflowAny) into work units {item, first run, end
 // run, path}: an item whose build side fits one fill of a block-shared table
 // (block_rcap tuples) is joined by join_block_kernel (path 0), a larger one
-// by join_general_kernel's wave tables with refills (path 1).  Waves take
-// tickets of 64 items and clear the flags they find, so the flags are all
-// zero again for the next join.
-// Runs and tuples of one side of item (p, q) for the calling lane (p and q
-// may differ between lanes): tuples = the sum of the item's run lengths.
+// by join_general_kernel's wave tables with refills (path 1).  Blocks take
+// tickets of 64 items, one item per lane; the waves of a block split the
+// items' run rows between them, so the partition of a heavy key (thousands
+// of pass-2 tiles, one run row each) is summed by the whole block.  Flags
+// found are cleared, so they are all zero again for the next join.
+constexpr uint32_t kUnitsWaves = 8;
+constexpr uint32_t kUnitsRows = 32;  // rows in flight per lane
+
+// Runs of one side of item (p, q) for the calling lane (p and q may differ
+// between lanes), and the sum of the lengths of its runs slice, slice +
+// kUnitsWaves, ... (in groups of kUnitsRows rows).
 __device__ __forceinline__ void lane_item_side(const PipelineDevice& d, const PipelineSide& s, uint32_t p,
-                                               uint32_t q, bool live, uint32_t& runs, uint64_t& tuples) {
+                                               uint32_t q, bool live, uint32_t slice, uint32_t& runs,
+                                               uint64_t& tuples) {
   runs = 0;
   tuples = 0;
   if (!d.bits2) {
     const uint32_t plen = live ? s.plen[p] : 0u;
     runs = (plen + d.tile2 - 1) / d.tile2;
-    tuples = plen;
+    tuples = slice == 0 ? plen : 0u;
     return;
   }
   const uint32_t t0 = s.tile_start[p];
@@ -673,39 +680,50 @@ __device__ __forceinline__ void lane_item_side(const PipelineDevice& d, const Pi
 #pragma unroll
   for (uint32_t off = 32; off > 0; off >>= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), off)));
   // coalesced across lanes where they share p: lane l reads word q of each row
-  for (uint32_t r = 0; r < most; r += 8) {
-    uint32_t w[8];
+  for (uint32_t r = slice * kUnitsRows; r < most; r += kUnitsWaves * kUnitsRows) {
+    uint32_t w[kUnitsRows];
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) {
+    for (uint32_t k = 0; k < kUnitsRows; ++k) {
       const uint32_t rr = min(r + k, runs ? runs - 1 : 0u);  // rows of an empty item: its spare row
       w[k] = s.meta[static_cast<uint64_t>(t0 + rr) * d.F2 + q];
     }
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k)
+    for (uint32_t k = 0; k < kUnitsRows; ++k)
       if (r + k < runs) tuples += (w[k] >> 16) - (w[k] & 0xffffu);
   }
 }
 
-__global__ __launch_bounds__(256) void join_units_kernel(PipelineDevice d, uint32_t block_rcap) {
+__global__ __launch_bounds__(kUnitsWaves * 64) void join_units_kernel(PipelineDevice d, uint32_t block_rcap) {
   if (uniform_load(d.work_counter + kOverflowAny) == 0) return;
-  const uint32_t lane = lane_id();
+  __shared__ uint32_t s_ticket;
+  __shared__ uint64_t part[2][kUnitsWaves][64];
+  const uint32_t lane = lane_id(), w = threadIdx.x / 64;
   const uint32_t F2 = d.bits2 ? d.F2 : 1;
   const uint32_t items = d.F1 * F2;
   for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(d.work_counter + kOverflowTicket, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
+    if (threadIdx.x == 0) s_ticket = atomicAdd(d.work_counter + kOverflowTicket, 1u);
+    __syncthreads();
+    const uint32_t t = s_ticket;
+    __syncthreads();  // every thread has read s_ticket
     if (static_cast<uint64_t>(t) * 64 >= items) break;
-    // one item per lane: its two sides' run counts and tuple counts
+    // one item per lane, the same in every wave
     const uint32_t item = min(t * 64 + lane, items - 1);
     const bool flagged = t * 64 + lane < items && d.overflow[item] != 0;
     if (__builtin_amdgcn_ballot_w64(flagged) == 0) continue;
-    if (flagged) d.overflow[item] = 0;
     const uint32_t p = item / F2, q = item % F2;
     uint32_t np, nb;
     uint64_t tuples, btuples;
-    lane_item_side(d, d.probe, p, q, flagged, np, tuples);
-    lane_item_side(d, d.build, p, q, flagged, nb, btuples);
+    lane_item_side(d, d.probe, p, q, flagged, w, np, tuples);
+    lane_item_side(d, d.build, p, q, flagged, w, nb, btuples);
+    part[0][w][lane] = tuples;
+    part[1][w][lane] = btuples;
+    __syncthreads();  // partial sums published; every wave has read its flag
+    if (w != 0) continue;
+    for (uint32_t i = 1; i < kUnitsWaves; ++i) {
+      tuples += part[0][i][lane];
+      btuples += part[1][i][lane];
+    }
+    if (flagged) d.overflow[item] = 0;
     const uint32_t path = btuples <= block_rcap ? 0u : 1u;
     uint32_t units = 0;
     if (flagged) units = max(1u, static_cast<uint32_t>(min<uint64_t>((tuples + kUnitTuples - 1) / kUnitTuples, np)));
@@ -793,14 +811,31 @@ __device__ __forceinline__ uint32_t publish_block_runs(const PipelineDevice& d, 
   return total;
 }
 
-// Tuple index of element e of a published block batch: the last run i with
-// prefix[i] <= e (empty runs share their successor's prefix and are passed).
-template <uint32_t BLOCK>
-__device__ __forceinline__ uint64_t block_element(const BlockRuns& br, uint32_t e) {
-  uint32_t l = 0;
+// First step of the search over a published batch of m runs (1 <= m <=
+// BLOCK): the largest power of two below m, 0 for m = 1.  The steps sum to at
+// most 2 * step - 1 < BLOCK, and every slot of the batch is written (slots at
+// or past m carry the batch total, above any element), so the search needs
+// no bound check and takes ceil(log2 m) LDS reads instead of log2 BLOCK.
+__device__ __forceinline__ uint32_t block_search_step(uint32_t m) {
+  return m > 1 ? 1u << (31 - __builtin_clz(m - 1)) : 0u;
+}
+
+// Tuple indices of elements e[0..N-1] of a published block batch: for each,
+// the last run i with prefix[i] <= e (empty runs share their successor's
+// prefix and are passed).  The N searches advance together, one step for all
+// of them per iteration, so their LDS reads are in flight at once.
+template <uint32_t N>
+__device__ __forceinline__ void block_elements(const BlockRuns& br, const uint32_t (&e)[N], uint32_t step0,
+                                               uint64_t (&idx)[N]) {
+  uint32_t l[N];
 #pragma unroll
-  for (uint32_t step = BLOCK / 2; step > 0; step >>= 1) l = br.prefix[l + step] <= e ? l + step : l;
-  return br.start[l] + (e - br.prefix[l]);
+  for (uint32_t k = 0; k < N; ++k) l[k] = 0;
+  for (uint32_t step = step0; step > 0; step >>= 1) {
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) l[k] = br.prefix[l[k] + step] <= e[k] ? l[k] + step : l[k];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < N; ++k) idx[k] = br.start[l[k]] + (e[k] - br.prefix[l[k]]);
 }
 
 // Joins the path-0 work units, one block per unit: the item's whole build
@@ -846,11 +881,16 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
       const uint32_t nb = run_count(d, d.build, p);
       for (uint32_t r0 = 0; r0 < nb; r0 += BLOCK) {
         const uint32_t tb = publish_block_runs<BLOCK>(d, d.build, p, q, r0, nb, br, wave_sums);
+        const uint32_t s0 = block_search_step(min(nb - r0, BLOCK));
         for (uint32_t e0 = 0; e0 < tb; e0 += 2 * BLOCK) {
+          uint32_t e[2];
+          uint64_t idx[2];
           T t[2];
 #pragma unroll
-          for (uint32_t k = 0; k < 2; ++k)
-            t[k] = load_tuple<K>(rel_b + block_element<BLOCK>(br, min(e0 + k * BLOCK + tid, tb - 1)));
+          for (uint32_t k = 0; k < 2; ++k) e[k] = min(e0 + k * BLOCK + tid, tb - 1);
+          block_elements(br, e, s0, idx);
+#pragma unroll
+          for (uint32_t k = 0; k < 2; ++k) t[k] = load_tuple<K>(rel_b + idx[k]);
 #pragma unroll
           for (uint32_t k = 0; k < 2; ++k)
             if (e0 + k * BLOCK + tid < tb && t[k].key != null_key<K>()) tab.insert(t[k].key, t[k].value);
@@ -859,11 +899,16 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
       }
       for (uint32_t r0 = pr0; r0 < pr1; r0 += BLOCK) {
         const uint32_t tp = publish_block_runs<BLOCK>(d, d.probe, p, q, r0, pr1, br, wave_sums);
+        const uint32_t s0 = block_search_step(min(pr1 - r0, BLOCK));
         for (uint32_t e0 = 0; e0 < tp; e0 += SPL * BLOCK) {
+          uint32_t e[SPL];
+          uint64_t idx[SPL];
           T t[SPL];
 #pragma unroll
-          for (uint32_t k = 0; k < SPL; ++k)
-            t[k] = load_tuple<K>(rel_p + block_element<BLOCK>(br, min(e0 + k * BLOCK + tid, tp - 1)));
+          for (uint32_t k = 0; k < SPL; ++k) e[k] = min(e0 + k * BLOCK + tid, tp - 1);
+          block_elements(br, e, s0, idx);
+#pragma unroll
+          for (uint32_t k = 0; k < SPL; ++k) t[k] = load_tuple<K>(rel_p + idx[k]);
 #pragma unroll
           for (uint32_t k = 0; k < SPL; ++k)
             if (e0 + k * BLOCK + tid < tp && t[k].key != null_key<K>()) tab.probe(t[k].key, t[k].value, cnt, sb, sp);
@@ -1143,7 +1188,7 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
   // block-shared tables: 64 KB, two blocks of 512 threads per CU
   constexpr uint32_t BCAP = 65536 / (2 * sizeof(K));
   constexpr uint32_t BBLOCK = 512;
-  hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(256), 0, stream, d, BCAP * 3 / 4);
+  hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(kUnitsWaves * 64), 0, stream, d, BCAP * 3 / 4);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   {
