@@ -780,11 +780,19 @@ __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice
 }
 
 // Block-wide batch of up to BLOCK runs of one side of an item, one run per
-// thread: run starts and the exclusive prefix of their lengths.
+// thread: run starts and the exclusive prefix of their lengths, and a bucket
+// table: bucket c holds the run of element c << shift (at most 2 * BLOCK
+// buckets), the slot after the last bucket holds the last run.
 struct BlockRuns {
   uint64_t* start;   // [BLOCK]
   uint32_t* prefix;  // [BLOCK]
+  uint16_t* bucket;  // [2 * BLOCK + 1]
 };
+
+template <uint32_t BLOCK>
+constexpr size_t block_runs_bytes() {
+  return BLOCK * (sizeof(uint64_t) + sizeof(uint32_t)) + (2 * BLOCK + 4) * sizeof(uint16_t);
+}
 
 // Publishes runs r0 .. r0 + BLOCK - 1 (those below nruns > r0) of item
 // (p, q) and returns their tuple count.  Ends with a barrier.
@@ -811,28 +819,72 @@ __device__ __forceinline__ uint32_t publish_block_runs(const PipelineDevice& d, 
   return total;
 }
 
-// First step of the search over a published batch of m runs (1 <= m <=
-// BLOCK): the largest power of two below m, 0 for m = 1.  The steps sum to at
-// most 2 * step - 1 < BLOCK, and every slot of the batch is written (slots at
-// or past m carry the batch total, above any element), so the search needs
-// no bound check and takes ceil(log2 m) LDS reads instead of log2 BLOCK.
+// First step of a binary search over m runs (1 <= m <= BLOCK): the largest
+// power of two below m, 0 for m = 1.  The steps sum to at most 2 * step - 1
+// < BLOCK, and every slot of a batch is written (slots at or past m carry the
+// batch total, above any element), so the search needs no bound check.
 __device__ __forceinline__ uint32_t block_search_step(uint32_t m) {
   return m > 1 ? 1u << (31 - __builtin_clz(m - 1)) : 0u;
 }
 
-// Tuple indices of elements e[0..N-1] of a published block batch: for each,
-// the last run i with prefix[i] <= e (empty runs share their successor's
-// prefix and are passed).  The N searches advance together, one step for all
-// of them per iteration, so their LDS reads are in flight at once.
-template <uint32_t N>
-__device__ __forceinline__ void block_elements(const BlockRuns& br, const uint32_t (&e)[N], uint32_t step0,
-                                               uint64_t (&idx)[N]) {
-  uint32_t l[N];
+// Fills the bucket table of a published batch of m runs holding n tuples and
+// returns its shift (buckets of 2^shift elements, at most 2 * BLOCK of them).
+// Each bucket is a binary search over the runs; the two buckets of a thread
+// search together.  Ends with a barrier.
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint32_t publish_block_buckets(const BlockRuns& br, uint32_t m, uint32_t n) {
+  constexpr uint32_t kLogBuckets = __builtin_ctz(2 * BLOCK);
+  static_assert((2 * BLOCK & (2 * BLOCK - 1)) == 0, "power-of-two block");
+  const uint32_t tid = threadIdx.x;
+  const uint32_t shift = n > 2 * BLOCK ? 32 - __builtin_clz(n - 1) - kLogBuckets : 0u;
+  const uint32_t nbk = (n + (1u << shift) - 1) >> shift;
+  const uint32_t step0 = block_search_step(m);
+  uint32_t pos[2], l[2] = {0, 0};
 #pragma unroll
-  for (uint32_t k = 0; k < N; ++k) l[k] = 0;
+  for (uint32_t k = 0; k < 2; ++k) pos[k] = (tid + k * BLOCK) << shift;
   for (uint32_t step = step0; step > 0; step >>= 1) {
 #pragma unroll
-    for (uint32_t k = 0; k < N; ++k) l[k] = br.prefix[l[k] + step] <= e[k] ? l[k] + step : l[k];
+    for (uint32_t k = 0; k < 2; ++k) l[k] = br.prefix[l[k] + step] <= pos[k] ? l[k] + step : l[k];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < 2; ++k)
+    if (tid + k * BLOCK < nbk) br.bucket[tid + k * BLOCK] = static_cast<uint16_t>(l[k]);
+  if (tid == 0) br.bucket[nbk] = static_cast<uint16_t>(m - 1);
+  __syncthreads();
+  return shift;
+}
+
+// Tuple indices of elements e[0..N-1] (each below the batch's tuple count) of
+// a published block batch: for each, the last run i with prefix[i] <= e
+// (empty runs share their successor's prefix and are passed).  The run lies
+// between the runs of the element's bucket and of the next bucket, and a
+// binary search over that range (usually zero to two runs wide) finds it.
+// The N searches take the same steps (sized for the widest range; a
+// candidate past a search's range is clamped to its end, which is then
+// accepted only if it is the run), and each step issues all N LDS reads
+// before it uses any.
+template <uint32_t N>
+__device__ __forceinline__ void block_elements(const BlockRuns& br, const uint32_t (&e)[N], uint32_t shift,
+                                               uint64_t (&idx)[N]) {
+  uint32_t l[N], hi[N];
+  uint32_t range = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < N; ++k) {
+    const uint32_t b = e[k] >> shift;
+    l[k] = br.bucket[b];
+    hi[k] = br.bucket[b + 1];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < N; ++k) range = max(range, hi[k] - l[k]);
+  for (uint32_t st = range ? 1u << (31 - __builtin_clz(range)) : 0u; st > 0; st >>= 1) {
+    uint32_t c[N], v[N];
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) c[k] = min(l[k] + st, hi[k]);
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) v[k] = br.prefix[c[k]];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) l[k] = v[k] <= e[k] ? c[k] : l[k];
   }
 #pragma unroll
   for (uint32_t k = 0; k < N; ++k) idx[k] = br.start[l[k]] + (e[k] - br.prefix[l[k]]);
@@ -853,8 +905,9 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
   __shared__ uint32_t wave_sums[BLOCK / 64];
   K* const tkeys = reinterpret_cast<K*>(lds_u64);
   const WaveTable<K, K, CAP> tab{tkeys, tkeys + CAP};
-  const BlockRuns br{reinterpret_cast<uint64_t*>(tkeys + 2 * CAP),
-                     reinterpret_cast<uint32_t*>(reinterpret_cast<uint64_t*>(tkeys + 2 * CAP) + BLOCK)};
+  uint64_t* const run_start = reinterpret_cast<uint64_t*>(tkeys + 2 * CAP);
+  uint32_t* const run_prefix = reinterpret_cast<uint32_t*>(run_start + BLOCK);
+  const BlockRuns br{run_start, run_prefix, reinterpret_cast<uint16_t*>(run_prefix + BLOCK)};
   const uint32_t tid = threadIdx.x;
   const uint32_t F2 = d.bits2 ? d.F2 : 1;
   const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
@@ -881,7 +934,7 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
       const uint32_t nb = run_count(d, d.build, p);
       for (uint32_t r0 = 0; r0 < nb; r0 += BLOCK) {
         const uint32_t tb = publish_block_runs<BLOCK>(d, d.build, p, q, r0, nb, br, wave_sums);
-        const uint32_t s0 = block_search_step(min(nb - r0, BLOCK));
+        const uint32_t s0 = publish_block_buckets<BLOCK>(br, min(nb - r0, BLOCK), tb);
         for (uint32_t e0 = 0; e0 < tb; e0 += 2 * BLOCK) {
           uint32_t e[2];
           uint64_t idx[2];
@@ -899,7 +952,7 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
       }
       for (uint32_t r0 = pr0; r0 < pr1; r0 += BLOCK) {
         const uint32_t tp = publish_block_runs<BLOCK>(d, d.probe, p, q, r0, pr1, br, wave_sums);
-        const uint32_t s0 = block_search_step(min(pr1 - r0, BLOCK));
+        const uint32_t s0 = publish_block_buckets<BLOCK>(br, min(pr1 - r0, BLOCK), tp);
         for (uint32_t e0 = 0; e0 < tp; e0 += SPL * BLOCK) {
           uint32_t e[SPL];
           uint64_t idx[SPL];
@@ -1193,7 +1246,7 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
   if (e != hipSuccess) return e;
   {
     auto blk = join_block_kernel<K, BCAP, BBLOCK, 16>;
-    const size_t lds = BCAP * 2 * sizeof(K) + BBLOCK * (8 + 4);
+    const size_t lds = BCAP * 2 * sizeof(K) + block_runs_bytes<BBLOCK>();
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(blk), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(lds));
     hipLaunchKernelGGL(blk, dim3(grid), dim3(BBLOCK), lds, stream, d);
