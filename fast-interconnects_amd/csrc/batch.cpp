@@ -178,6 +178,7 @@ fij_status grow_arena(ClassArena& a, KeyType kt, uint32_t b1, uint32_t b2, uint6
   if (e == hipSuccess) e = arena_alloc(n, &d.work_counter, kJoinCounterWords);
   if (e == hipSuccess) e = arena_alloc(n, &d.overflow, parts * F2);
   if (e == hipSuccess) e = arena_alloc(n, &d.units, n.units);
+  d.unit_cap = static_cast<uint32_t>(n.units);
   if (e == hipSuccess) e = arena_alloc(n, &d.aggregates, 1);
   if (e == hipSuccess) e = arena_alloc(n, &n.hist, 2 * n.joins);
   if (e == hipSuccess) e = arena_alloc(n, &n.scan, 2 * n.joins);

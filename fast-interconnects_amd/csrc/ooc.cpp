@@ -359,7 +359,8 @@ extern "C" fij_status fij_ooc_join(const fij_ooc_config* cfg, const void* build_
     const uint64_t items = static_cast<uint64_t>(F1) * F2;
     OOC_HIP(A.dalloc(&d.overflow, items));
     OOC_HIP(hipMemsetAsync(d.overflow, 0, items, st[s]));
-    OOC_HIP(A.dalloc(&d.units, items + 2 * cap / 65536 + 2));
+    d.unit_cap = static_cast<uint32_t>(items + 2 * cap / 65536 + 2);
+    OOC_HIP(A.dalloc(&d.units, d.unit_cap));
     OOC_HIP(A.dalloc(&d.aggregates, join_grid));
   }
   for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamSynchronize(st[s]));

@@ -727,20 +727,30 @@ __global__ __launch_bounds__(kUnitsWaves * 64) void join_units_kernel(PipelineDe
     const uint32_t path = btuples <= block_rcap ? 0u : 1u;
     uint32_t units = 0;
     if (flagged) units = max(1u, static_cast<uint32_t>(min<uint64_t>((tuples + kUnitTuples - 1) / kUnitTuples, np)));
-    const uint32_t incl = wave_inclusive_scan(units);
-    uint32_t base = 0;
-    if (lane == 63) base = atomicAdd(d.work_counter + kUnitCount, incl);
-    base = __builtin_amdgcn_readlane(base, 63) + incl - units;
+    // block-table units go to the front of `units`, wave-table units to the
+    // back (unit_cap bounds both together), so each join kernel visits only
+    // its own
+    const uint32_t ub = path == 0 ? units : 0u, uw = path == 0 ? 0u : units;
+    const uint32_t incl_b = wave_inclusive_scan(ub), incl_w = wave_inclusive_scan(uw);
+    uint32_t base_b = 0, base_w = 0;
+    if (lane == 63) {
+      if (incl_b) base_b = atomicAdd(d.work_counter + kUnitCount, incl_b);
+      if (incl_w) base_w = atomicAdd(d.work_counter + kWaveUnitCount, incl_w);
+    }
+    base_b = __builtin_amdgcn_readlane(base_b, 63) + incl_b - ub;
+    base_w = __builtin_amdgcn_readlane(base_w, 63) + incl_w - uw;
     for (uint32_t u = 0; u < units; ++u) {
       const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(u) * np / units);
       const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(u + 1) * np / units);
-      d.units[base + u] = make_uint4(t * 64 + lane, r0, r1, path);
+      const uint32_t slot = path == 0 ? base_b + u : d.unit_cap - 1 - (base_w + u);
+      d.units[slot] = make_uint4(t * 64 + lane, r0, r1, path);
     }
   }
 }
 
-// Joins the work units (one wave per unit, tickets of up to 8 units).  When
-// no item was deferred every block only adds a zero aggregate.
+// Joins the wave-table work units (path 1, stored from the back of `units`;
+// one wave per unit, tickets of up to 8 units).  When no item was deferred
+// every block only adds a zero aggregate.
 template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL>
 __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice d) {
   using T = Tuple<K>;
@@ -752,7 +762,7 @@ __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice
   const T* rel_b = static_cast<const T*>(d.bits2 ? d.build.rel2 : d.build.rel1);
   const T* rel_p = static_cast<const T*>(d.bits2 ? d.probe.rel2 : d.probe.rel1);
   uint64_t cnt = 0, sb = 0, sp = 0;
-  const uint32_t n = uniform_load(d.work_counter + kOverflowAny) ? uniform_load(d.work_counter + kUnitCount) : 0;
+  const uint32_t n = uniform_load(d.work_counter + kOverflowAny) ? uniform_load(d.work_counter + kWaveUnitCount) : 0;
   if (n) {
     WaveTable<K, K, CAP>{l.table_keys, l.table_keys + CAP}.clear();
     wave_lds_sync();
@@ -766,8 +776,7 @@ __global__ __launch_bounds__(WAVES * 64) void join_general_kernel(PipelineDevice
       if (t >= n) break;
       const uint32_t e = min(t + tk, n);
       for (uint32_t u = t; u < e; ++u) {
-        const uint4 unit = d.units[u];
-        if (__builtin_amdgcn_readfirstlane(unit.w) == 0) continue;  // joined by join_block_kernel
+        const uint4 unit = d.units[d.unit_cap - 1 - u];
         const uint32_t item = __builtin_amdgcn_readfirstlane(unit.x);
         join_item_general<K, CAP, RCAP, BPL, SPL>(d, item / F2, item % F2, __builtin_amdgcn_readfirstlane(unit.y),
                                                   __builtin_amdgcn_readfirstlane(unit.z), l.table_keys, l.rb, l.rp,
@@ -927,7 +936,6 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
       __syncthreads();  // every thread has read s_unit
       if (u >= n) break;
       const uint4 unit = d.units[u];
-      if (__builtin_amdgcn_readfirstlane(unit.w) != 0) continue;  // wave-table unit
       const uint32_t item = __builtin_amdgcn_readfirstlane(unit.x);
       const uint32_t p = item / F2, q = item % F2;
       const uint32_t pr0 = __builtin_amdgcn_readfirstlane(unit.y), pr1 = __builtin_amdgcn_readfirstlane(unit.z);

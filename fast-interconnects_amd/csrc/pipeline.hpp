@@ -37,8 +37,10 @@ struct PipelineSide {
 // Join work counters, each on its own 128-byte line: one per XCD (work
 // groups are dispatched to the 8 XCDs round-robin), then the ticket counter
 // over the deferred-item flags, a word that is non-zero when any item was
-// deferred, the number of work units made of them, the ticket counter of the
-// wave-table units and that of the block-table units.
+// deferred, the number of block-table work units made of them (stored from
+// the front of `units`), the ticket counter of the wave-table units, that of
+// the block-table units, and the number of wave-table units (stored from the
+// back of `units`).
 constexpr uint32_t kJoinCounters = 8;
 constexpr uint32_t kJoinCounterStride = 32;
 constexpr uint32_t kOverflowTicket = kJoinCounters * kJoinCounterStride;
@@ -46,7 +48,8 @@ constexpr uint32_t kOverflowAny = (kJoinCounters + 1) * kJoinCounterStride;
 constexpr uint32_t kUnitCount = (kJoinCounters + 2) * kJoinCounterStride;
 constexpr uint32_t kUnitTicket = (kJoinCounters + 3) * kJoinCounterStride;
 constexpr uint32_t kBlockTicket = (kJoinCounters + 4) * kJoinCounterStride;
-constexpr uint32_t kJoinCounterWords = (kJoinCounters + 5) * kJoinCounterStride;
+constexpr uint32_t kWaveUnitCount = (kJoinCounters + 5) * kJoinCounterStride;
+constexpr uint32_t kJoinCounterWords = (kJoinCounters + 6) * kJoinCounterStride;
 
 struct PipelineDevice {
   PipelineSide build;
@@ -58,7 +61,8 @@ struct PipelineDevice {
   uint32_t sides;             // bit 0: build, bit 1: probe -- the sides the plan and pass-2 kernels process
   uint32_t* work_counter;     // [kJoinCounterWords]
   uint8_t* overflow;          // [F1 * F2] 1 = item deferred to the general join (zero between joins)
-  uint4* units;               // [unit capacity] {item, first probe run, end probe run, 1 = wave table}
+  uint4* units;               // [unit_cap] {item, first probe run, end probe run, 1 = wave table}
+  uint32_t unit_cap;          // at least the items plus one per 65,536 probe tuples
   JoinAggregate* aggregates;  // [agg_len]
   uint32_t agg_len;
   // Many joins in one plan (batched joins): pass-1 partition p belongs to

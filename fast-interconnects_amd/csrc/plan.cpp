@@ -160,7 +160,8 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
   if (st == FIJ_OK) st = alloc(p, &p->d.overflow, items);
   // work units of deferred items: at most one per item plus one per 65536
   // probe tuples
-  if (st == FIJ_OK) st = alloc(p, &p->d.units, items + cfg->probe_len / 65536 + 1);
+  p->d.unit_cap = static_cast<uint32_t>(items + cfg->probe_len / 65536 + 1);
+  if (st == FIJ_OK) st = alloc(p, &p->d.units, p->d.unit_cap);
   // the deferred-item flags start at zero (complete before any stream uses
   // them); every join leaves them at zero
   if (st == FIJ_OK && (hipMemset(p->d.overflow, 0, items) != hipSuccess || hipDeviceSynchronize() != hipSuccess))

@@ -96,6 +96,7 @@ struct Slot {
   uint32_t* work_counter = nullptr;
   uint8_t* overflow = nullptr;
   uint4* units = nullptr;
+  uint32_t unit_cap = 0;
   JoinAggregate* aggregates = nullptr;
   JoinAggregate* d_result = nullptr;
   JoinAggregate* h_result = nullptr;
@@ -184,6 +185,7 @@ fij_status launch_request(fij_join_server* s, Slot& sl, const Request& r) {
   d.work_counter = sl.work_counter;
   d.overflow = sl.overflow;
   d.units = sl.units;
+  d.unit_cap = sl.unit_cap;
   d.aggregates = sl.aggregates;
   hipStream_t st = sl.stream;
   SRV_HIP(enqueue_side_prefix_sum(d, d.probe, s->kt, 1024, st));
@@ -277,7 +279,8 @@ fij_status make_slot(fij_join_server* s, Slot& sl, uint64_t probe_cap) {
   SRV_HIP(alloc(reinterpret_cast<void**>(&sl.work_counter), kJoinCounterWords * 4));
   SRV_HIP(alloc(reinterpret_cast<void**>(&sl.overflow), items));
   SRV_HIP(hipMemset(sl.overflow, 0, items));
-  SRV_HIP(alloc(reinterpret_cast<void**>(&sl.units), (items + probe_cap / 65536 + 1) * sizeof(uint4)));
+  sl.unit_cap = static_cast<uint32_t>(items + probe_cap / 65536 + 1);
+  SRV_HIP(alloc(reinterpret_cast<void**>(&sl.units), sl.unit_cap * sizeof(uint4)));
   SRV_HIP(alloc(reinterpret_cast<void**>(&sl.aggregates), s->join_grid * sizeof(JoinAggregate)));
   SRV_HIP(alloc(reinterpret_cast<void**>(&sl.d_result), sizeof(JoinAggregate)));
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_result), sizeof(JoinAggregate), hipHostMallocDefault));
