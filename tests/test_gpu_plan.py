@@ -117,3 +117,19 @@ def test_plan_skewed_probe():
     sp = orc.gen_iota(4 * n, 0, np.int64)
     got, _ = run_plan(rk, rp, sk, sp, 9, 9)
     assert got == orc.sort_merge_join(rk, rp, sk, sp)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_plan_deferred_many_runs(dtype):
+    """A 2^24 probe side over two pass-1 partitions (1 + 12 bits): each of an
+    item's 1,024 probe runs (one per pass-2 tile) holds about two tuples, so
+    every item is deferred and the block join takes its runs in two batches
+    of 512; its 16 build runs form one batch.  Closed form: every probe key
+    is one of R's keys, and R's payload is its key."""
+    n_r, n_s = 1 << 18, 1 << 24
+    rk = orc.gen_perm(n_r, 21, dtype, 1)
+    rp = rk.copy()
+    sk = orc.gen_uniform(n_s, 21, 1, 1, n_r, dtype)
+    sp = orc.gen_iota(n_s, 0, dtype)
+    got, _ = run_plan(rk, rp, sk, sp, 1, 12)
+    assert got == (n_s, int(sk.astype(np.int64).sum()), n_s * (n_s - 1) // 2)
