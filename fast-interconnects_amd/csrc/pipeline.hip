@@ -19,7 +19,9 @@
 //
 // With bits2 == 0 (single pass) the join reads the pass-1 chunk pieces
 // directly.  Work items of the join are taken from an atomic counter so that
-// skewed partitions do not serialise a fixed wave.
+// skewed partitions do not serialise a fixed wave; pairs too large for a
+// wave's registers are deferred, split into work units and joined from
+// block-shared (join_block_kernel) or wave LDS tables (join_general_kernel).
 #include <algorithm>
 #include <type_traits>
 #include <vector>
@@ -1106,8 +1108,9 @@ __global__ __launch_bounds__(WAVES * 64) void join_kernel(PipelineDevice d) {
         gp = rel_p;
         staged = true;
       } else if (!empty && lane == 0) {
-        // does not fit the registers: deferred to join_units_kernel and
-        // join_general_kernel (an empty item has no match)
+        // does not fit the registers: deferred to join_units_kernel, which
+        // hands it to join_block_kernel or join_general_kernel (an empty
+        // item has no match)
         d.overflow[cur] = 1;
         d.work_counter[kOverflowAny] = 1;
       }
