@@ -12,9 +12,16 @@
 //
 // Each thread histograms its chunk; the counts are scanned into partition
 // and per-thread offsets; each thread then scatters its chunk through
-// per-partition 64-byte write-combining buffers.  Order within a partition
-// is that of the threads, each thread's tuples in input order.
+// per-partition 64-byte write-combining buffers, each standing for one
+// 64-byte line of the output.  A full line is written with non-temporal
+// stores (no read for ownership); the partial lines at either end of a
+// thread's range in a partition, which other threads or padding share, are
+// copied.  Order within a partition is that of the threads, each thread's
+// tuples in input order.
+#include <emmintrin.h>
+
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -75,23 +82,38 @@ void partition_t(const K* keys, const K* pays, uint64_t n, uint32_t bits, uint32
       run += cnt;
     }
   }
+  // lines are 64-byte aligned in memory only when the output is
+  const bool lines = reinterpret_cast<uintptr_t>(out) % 64 == 0;
   parallel_for(threads, [&](uint32_t t) {
     const uint64_t b = std::min(n, per * t), e = std::min(n, b + per);
-    uint64_t* cur = &hist[static_cast<size_t>(t) * F];
-    std::vector<T> buf(static_cast<size_t>(F) * L);
-    std::vector<uint32_t> fill(F, 0);
+    uint64_t* cur = &hist[static_cast<size_t>(t) * F];  // first tuple of the buffered line
+    T* buf = static_cast<T*>(std::aligned_alloc(64, static_cast<size_t>(F) * 64));
+    std::vector<uint32_t> fill(F), lo(F);  // buffered slots [lo, fill) of the line
+    for (uint32_t d = 0; d < F; ++d) {
+      lo[d] = fill[d] = lines ? static_cast<uint32_t>(cur[d] % L) : 0u;
+      cur[d] -= lo[d];
+    }
     for (uint64_t i = b; i < e; ++i) {
       const uint32_t d = host_digit(keys[i], mask, ignore);
-      T* slot = &buf[static_cast<size_t>(d) * L];
+      T* slot = buf + static_cast<size_t>(d) * L;
       slot[fill[d]] = T{keys[i], pays[i]};
       if (++fill[d] == L) {
-        memcpy(out + cur[d], slot, sizeof(T) * L);
+        if (lo[d] == 0 && lines) {
+          const __m128i* src = reinterpret_cast<const __m128i*>(slot);
+          __m128i* dst = reinterpret_cast<__m128i*>(out + cur[d]);
+          for (uint32_t k = 0; k < 4; ++k) _mm_stream_si128(dst + k, _mm_load_si128(src + k));
+        } else {
+          memcpy(out + cur[d] + lo[d], slot + lo[d], sizeof(T) * (L - lo[d]));
+        }
         cur[d] += L;
-        fill[d] = 0;
+        fill[d] = lo[d] = 0;
       }
     }
     for (uint32_t d = 0; d < F; ++d)
-      if (fill[d]) memcpy(out + cur[d], &buf[static_cast<size_t>(d) * L], sizeof(T) * fill[d]);
+      if (fill[d] > lo[d])
+        memcpy(out + cur[d] + lo[d], buf + static_cast<size_t>(d) * L + lo[d], sizeof(T) * (fill[d] - lo[d]));
+    _mm_sfence();  // the streamed lines are visible before the threads join
+    std::free(buf);
   });
 }
 

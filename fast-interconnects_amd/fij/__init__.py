@@ -1045,15 +1045,21 @@ def cpu_prefix_sum(histogram_algorithm, keys, radix_bits, chunks, ignore_bits=0,
     return (offs, local) if contiguous else offs
 
 
-def cpu_radix_partition(keys, payloads, radix_bits, ignore_bits=0, threads=0):
+def cpu_radix_partition(keys, payloads, radix_bits, ignore_bits=0, threads=0, out=None):
     """Contiguous pass 1 on the host: (keys, values) of the padded relation
     (padding slots hold whatever was there: zeros here) and the partition
-    starts."""
+    starts.  `out`: a C-contiguous (len + F * pad, 2) array of the key type
+    to write into instead of a new zeroed one."""
     keys = np.ascontiguousarray(keys)
     payloads = np.ascontiguousarray(payloads, dtype=keys.dtype)
     F = 1 << radix_bits
     pad = 128 // (2 * keys.dtype.itemsize)
-    rel = np.zeros((len(keys) + F * pad, 2), keys.dtype)
+    if out is None:
+        rel = np.zeros((len(keys) + F * pad, 2), keys.dtype)
+    else:
+        rel = out
+        if rel.shape != (len(keys) + F * pad, 2) or rel.dtype != keys.dtype or not rel.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous (len + F * pad, 2) array of the key type")
     offs = np.zeros(F, np.uint64)
     check(lib.fij_cpu_radix_partition(int(_ktype(keys.dtype)), keys.ctypes.data_as(_vp),
                                       payloads.ctypes.data_as(_vp), len(keys), radix_bits, ignore_bits, threads,
