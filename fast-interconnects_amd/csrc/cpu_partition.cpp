@@ -21,7 +21,6 @@
 #include <emmintrin.h>
 
 #include <algorithm>
-#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -87,7 +86,8 @@ void partition_t(const K* keys, const K* pays, uint64_t n, uint32_t bits, uint32
   parallel_for(threads, [&](uint32_t t) {
     const uint64_t b = std::min(n, per * t), e = std::min(n, b + per);
     uint64_t* cur = &hist[static_cast<size_t>(t) * F];  // first tuple of the buffered line
-    T* buf = static_cast<T*>(std::aligned_alloc(64, static_cast<size_t>(F) * 64));
+    std::vector<uint8_t> storage(static_cast<size_t>(F) * 64 + 64);  // F lines, 64-byte aligned
+    T* buf = reinterpret_cast<T*>(storage.data() + (-reinterpret_cast<uintptr_t>(storage.data()) & 63));
     std::vector<uint32_t> fill(F), lo(F);  // buffered slots [lo, fill) of the line
     for (uint32_t d = 0; d < F; ++d) {
       lo[d] = fill[d] = lines ? static_cast<uint32_t>(cur[d] % L) : 0u;
@@ -113,7 +113,6 @@ void partition_t(const K* keys, const K* pays, uint64_t n, uint32_t bits, uint32
       if (fill[d] > lo[d])
         memcpy(out + cur[d] + lo[d], buf + static_cast<size_t>(d) * L + lo[d], sizeof(T) * (fill[d] - lo[d]));
     _mm_sfence();  // the streamed lines are visible before the threads join
-    std::free(buf);
   });
 }
 
