@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--cpu-log2n", type=int, default=23, help="CPU baseline sample size (log2 of |R| = |S|)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--verify", action="store_true", help="check the result against the oracle at full size")
-    ap.add_argument("--workload", choices=["plain", "skew", "batched", "served", "ooc", "np"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "skew", "batched", "served", "ooc", "np", "operator"],
+                    default="plain",
                     help="plain: R = S = 2^27 int64, 9+9 bits (the headline line); skew: R = 2^27 int32 unique, "
                          "S = 2^30 int32 Zipf over R, 8+8 bits; batched: 4096 int32 joins of log-uniform sizes")
     ap.add_argument("--zipf", type=float, default=1.0, help="skew: Zipf exponent")
@@ -531,6 +532,111 @@ def np_main(a):
     print(json.dumps(out))
 
 
+def operator_main(a):
+    """The operator API the reference's radix-join binary drives, one op at a
+    time on one stream, timed by HIP events: chunked and contiguous prefix
+    sums, prefix sum with payload copy, the five scatter variants, task
+    assignment, and the LDS perfect and bucket-chaining joins.  Shape: R =
+    permutation of 1..2^20 (payload = key), S = 2^24 keys uniform over R's
+    keys (payload = row index), int32, one pass of 8 bits; partitioner grid
+    = one chunk per CU, join grid = one block per CU.  `value`: R and S
+    tuples per second of contiguous prefix sums + scatters (SSWWCv2) +
+    chaining join (the join takes single-chunk relations)."""
+    import fij
+
+    cus = fij.device_limits()["compute_units"]
+    n_r, n_s, bits = 1 << 20, 1 << 24, 8
+    dt, kt, k = np.int32, fij.KeyType.I32, 4
+    t = 2 * k
+    rk, rp = fij.DeviceArray(n_r, dt), fij.DeviceArray(n_r, dt)
+    sk, sp = fij.DeviceArray(n_s, dt), fij.DeviceArray(n_s, dt)
+    fij.gen_permutation(rk, SEED, 1)
+    fij.check(fij.lib.fij_memcpy(fij.C.c_void_p(rp.ptr), fij.C.c_void_p(rk.ptr), rk.nbytes(), None))
+    fij.gen_uniform(sk, SEED, 1, 1, n_r)
+    fij.gen_iota(sp, 0)
+    fij.synchronize()
+    st = fij.Stream()
+    e0, e1 = fij.Event(), fij.Event()
+    CH, CO = fij.HistogramAlgorithm.Chunked, fij.HistogramAlgorithm.Contiguous
+    FIRST = fij.RadixPass.First
+
+    def timed(op, before=None):
+        ts = []
+        for i in range(a.warmup + a.steps):
+            if before:
+                before()
+            e0.record(st)
+            op()
+            e1.record(st)
+            st.synchronize()
+            if i >= a.warmup:
+                ts.append(e1.elapsed_ms(e0))
+        return float(np.median(ts))
+
+    def line(ms, nbytes):
+        gbs = nbytes / (ms / 1000.0) / 1e9
+        return {"ms": ms, "bytes": nbytes, "gbs": gbs, "frac": gbs / 8000.0}
+
+    ops = {}
+    for name, alg in (("prefix_sum_chunked", CH), ("prefix_sum_contiguous", CO)):
+        part = fij.GpuRadixPartitioner(alg, fij.RadixPartitionAlgorithm.SSWWCv2, fij.RadixBits(bits), cus, 1024)
+        offs = fij.PartitionOffsets(alg, cus, bits, kt)
+        ops[name] = line(timed(lambda: part.prefix_sum(FIRST, sk, offs, stream=st)), n_s * k)
+    part = fij.GpuRadixPartitioner(CO, fij.RadixPartitionAlgorithm.SSWWCv2, fij.RadixBits(bits), cus, 1024)
+    offs = fij.PartitionOffsets(CO, cus, bits, kt)
+    dk, dp = fij.DeviceArray(n_s, dt), fij.DeviceArray(n_s, dt)
+    ops["prefix_sum_and_copy_with_payload"] = line(
+        timed(lambda: part.prefix_sum_and_copy_with_payload(FIRST, sk, sp, dk, dp, offs, stream=st)), n_s * 4 * k)
+    del dk, dp
+
+    def partitioned(keys, pays, alg, block, hist=CH):
+        """(partitioner, offsets, relation, timed scatter ms): every timed
+        scatter follows its own untimed prefix sum (a scatter moves the
+        offsets into the relation)."""
+        pt = fij.GpuRadixPartitioner(hist, alg, fij.RadixBits(bits), cus, block)
+        of = fij.PartitionOffsets(hist, cus, bits, kt)
+        rel = fij.PartitionedRelation(len(keys), hist, bits, cus, kt)
+        ms = timed(lambda: pt.partition(FIRST, keys, pays, of, rel, stream=st),
+                   before=lambda: pt.prefix_sum(FIRST, keys, of, stream=st))
+        return pt, of, rel, ms
+
+    for alg in fij.RadixPartitionAlgorithm:
+        block = 512 if alg == fij.RadixPartitionAlgorithm.HSSWWCv4 else 1024
+        ms = partitioned(sk, sp, alg, block)[3]
+        ops[f"partition_{alg.name}"] = line(ms, n_s * (2 * k + t))
+    # the join's inputs: contiguous layout (one range per partition), as
+    # the join and its task assignment take single-chunk relations
+    psum_r = timed(lambda: part.prefix_sum(FIRST, rk, fij.PartitionOffsets(CO, cus, bits, kt), stream=st))
+    _, _, brel, part_r = partitioned(rk, rp, fij.RadixPartitionAlgorithm.SSWWCv2, 1024, CO)
+    _, _, prel, part_s = partitioned(sk, sp, fij.RadixPartitionAlgorithm.SSWWCv2, 1024, CO)
+    tasks = fij.DeviceArray.zeros(cus + 1, np.uint32)
+    ops["assign_tasks"] = line(timed(lambda: fij.assign_tasks(prel, bits, cus, tasks, stream=st)), 8 * (1 << bits))
+    want_build = int(sk.to_host().astype(np.int64).sum())
+    joins = {}
+    for scheme in (fij.HashingScheme.Perfect, fij.HashingScheme.BucketChaining):
+        j = fij.HipRadixJoin(FIRST, fij.RadixBits(bits), scheme, cus, 1024)
+        agg = fij.aggregates_array(cus)
+        ms = timed(lambda: j.join(brel, prel, None, tasks, agg, stream=st), before=lambda: agg.fill(0, stream=st))
+        got = fij.sum_aggregates(agg)
+        assert got == (n_s, want_build, n_s * (n_s - 1) // 2), f"{scheme.name} join result {got}"
+        ops[f"join_{scheme.name}"] = line(ms, (n_r + n_s) * t)
+        joins[scheme.name] = ms
+    total = ops["prefix_sum_contiguous"]["ms"] + psum_r + part_r + part_s + joins["BucketChaining"]
+    out = {
+        "metric": METRIC, "value": (n_r + n_s) / (total / 1000.0), "unit": "tuples/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": total, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded on-device generators)",
+        "config": {"workload": "operator API, one pass of 8 bits: R = permutation of 1..2^20 int32 (payload = key), "
+                               "S = 2^24 uniform over R's keys; contiguous prefix sums + SSWWCv2 scatters + "
+                               "chaining join",
+                   "partitioner_grid": cus, "join_grid": cus, "parallelism": "single GPU"},
+        "ops": ops,
+        "note": "bytes per op: prefix sums N k, copy with payload N 4k, scatters N (2k + t), joins (R + S) t, "
+                "assign_tasks 8 B per partition; frac = GB/s / 8000",
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -547,6 +653,8 @@ def main():
         return ooc_main(a)
     if a.workload == "np":
         return np_main(a)
+    if a.workload == "operator":
+        return operator_main(a)
     import fij
 
     n = 1 << a.log2n
