@@ -45,22 +45,38 @@ __global__ __launch_bounds__(1024) void assign_tasks_kernel(const uint64_t* prob
     sizes[p] = static_cast<uint32_t>(upper - probe_offsets[p]);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  // The greedy rule (a task ends at the first partition whose size brings
+  // the task to at least avg probe tuples) taken 64 partitions at a time by
+  // wave 0: an inclusive scan of the sizes gives every lane the running sum
+  // since the last cut, a ballot finds the next cut; one step per cut.
+  if (threadIdx.x < kWave) {
+    const uint32_t lane = threadIdx.x;
     const uint32_t probe_tuples = probe_len - pad * F;
     const uint32_t avg = (probe_tuples + grid - 1u) / grid;
-    ta[0] = 0;
-    uint32_t task_id = 1, task_size = 0;
-    for (uint32_t p = 0; p < F && task_id < grid; ++p) {
-      task_size += sizes[p];
-      if (task_size >= avg) {
-        ta[task_id] = p + 1;
-        task_size = 0;
+    if (lane == 0) ta[0] = 0;
+    uint32_t task_id = 1, task_size = 0;  // wave-uniform
+    for (uint32_t base = 0; base < F && task_id < grid; base += kWave) {
+      const bool valid = base + lane < F;
+      const uint32_t incl = wave_inclusive_scan(valid ? sizes[base + lane] : 0u);
+      uint32_t carry = task_size, before = 0, from = 0;  // running sum at lane l: carry + incl[l] - before
+      while (task_id < grid) {
+        const uint64_t hit = ballot(lane >= from && valid && carry + incl - before >= avg);
+        if (hit == 0) break;
+        const uint32_t l = static_cast<uint32_t>(__builtin_ctzll(hit));
+        if (lane == 0) ta[task_id] = base + l + 1;
         task_id += 1;
+        carry = 0;
+        before = __builtin_amdgcn_readlane(incl, l);
+        from = l + 1;
       }
+      task_size = carry + __builtin_amdgcn_readlane(incl, kWave - 1) - before;
     }
-    for (uint32_t t = task_id; t <= grid; ++t) ta[t] = F;
+    for (uint32_t t = task_id + lane; t <= grid; t += kWave) ta[t] = F;
   }
 }
+
+// Probe tuples in flight per thread in the LDS joins.
+constexpr uint32_t kJoinLoads = 8;
 
 __device__ __forceinline__ void partition_range(const uint64_t* offsets, uint64_t padded_len, uint32_t F,
                                                 uint32_t p, uint32_t pad, uint64_t& begin, uint64_t& end) {
@@ -128,14 +144,23 @@ __global__ __launch_bounds__(1024) void join_perfect_kernel(JoinParams p) {
       else overflow = true;
     }
     __syncthreads();
-    for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
-      const Tuple<K> t = load_tuple<K>(probe + i);
-      const uint32_t idx = perfect_index<K>(t.key, p.ignore_bits);
-      if (t.key != null_key<K>() && idx < E && ht[idx].key == t.key) {
-        cnt += 1;
-        sb += static_cast<uint64_t>(static_cast<int64_t>(ht[idx].value));
-        sp += static_cast<uint64_t>(static_cast<int64_t>(t.value));
-        ref_sum += static_cast<int64_t>(t.value);
+    // thread i takes tuples q0 + i, q0 + i + blockDim, ... (kJoinLoads loads
+    // in flight; lanes past the end re-read the last tuple and skip it)
+    for (uint64_t i0 = q0; i0 < q1; i0 += kJoinLoads * blockDim.x) {
+      Tuple<K> t[kJoinLoads];
+#pragma unroll
+      for (uint32_t k = 0; k < kJoinLoads; ++k)
+        t[k] = load_tuple<K>(probe + min<uint64_t>(i0 + k * blockDim.x + threadIdx.x, q1 - 1));
+#pragma unroll
+      for (uint32_t k = 0; k < kJoinLoads; ++k) {
+        const uint32_t idx = perfect_index<K>(t[k].key, p.ignore_bits);
+        if (i0 + k * blockDim.x + threadIdx.x < q1 && t[k].key != null_key<K>() && idx < E &&
+            ht[idx].key == t[k].key) {
+          cnt += 1;
+          sb += static_cast<uint64_t>(static_cast<int64_t>(ht[idx].value));
+          sp += static_cast<uint64_t>(static_cast<int64_t>(t[k].value));
+          ref_sum += static_cast<int64_t>(t[k].value);
+        }
       }
     }
     __syncthreads();
@@ -179,16 +204,22 @@ __global__ __launch_bounds__(1024) void join_chaining_kernel(JoinParams p) {
       links[i] = static_cast<uint16_t>(atomicExch(&heads[bucket], i));
     }
     __syncthreads();
-    for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
-      const Tuple<K> t = load_tuple<K>(probe + i);
-      if (t.key == null_key<K>()) continue;
-      const uint32_t bucket = mult_shift_hash32(perfect_index<K>(t.key, p.ignore_bits), log2_buckets);
-      for (uint32_t e = heads[bucket]; e != kChainTail; e = links[e]) {
-        if (keys[e] == t.key) {
-          cnt += 1;
-          sb += static_cast<uint64_t>(static_cast<int64_t>(vals[e]));
-          sp += static_cast<uint64_t>(static_cast<int64_t>(t.value));
-          ref_sum += static_cast<int64_t>(t.value);
+    for (uint64_t i0 = q0; i0 < q1; i0 += kJoinLoads * blockDim.x) {
+      Tuple<K> t[kJoinLoads];
+#pragma unroll
+      for (uint32_t k = 0; k < kJoinLoads; ++k)
+        t[k] = load_tuple<K>(probe + min<uint64_t>(i0 + k * blockDim.x + threadIdx.x, q1 - 1));
+#pragma unroll
+      for (uint32_t k = 0; k < kJoinLoads; ++k) {
+        if (i0 + k * blockDim.x + threadIdx.x >= q1 || t[k].key == null_key<K>()) continue;
+        const uint32_t bucket = mult_shift_hash32(perfect_index<K>(t[k].key, p.ignore_bits), log2_buckets);
+        for (uint32_t e = heads[bucket]; e != kChainTail; e = links[e]) {
+          if (keys[e] == t[k].key) {
+            cnt += 1;
+            sb += static_cast<uint64_t>(static_cast<int64_t>(vals[e]));
+            sp += static_cast<uint64_t>(static_cast<int64_t>(t[k].value));
+            ref_sum += static_cast<int64_t>(t[k].value);
+          }
         }
       }
     }
