@@ -539,7 +539,9 @@ def operator_main(a):
     assignment, and the LDS perfect and bucket-chaining joins.  Shape: R =
     permutation of 1..2^20 (payload = key), S = 2^24 keys uniform over R's
     keys (payload = row index), int32, one pass of 8 bits; partitioner grid
-    = one chunk per CU, join grid = one block per CU.  `value`: R and S
+    = one chunk per CU, join grid = two blocks per CU (the greedy task rule
+    then gives every partition a task of its own; with one block per CU a
+    partition a little below the average size pulls in the next one).  `value`: R and S
     tuples per second of contiguous prefix sums + scatters (SSWWCv2) +
     chaining join (the join takes single-chunk relations)."""
     import fij
@@ -609,13 +611,14 @@ def operator_main(a):
     psum_r = timed(lambda: part.prefix_sum(FIRST, rk, fij.PartitionOffsets(CO, cus, bits, kt), stream=st))
     _, _, brel, part_r = partitioned(rk, rp, fij.RadixPartitionAlgorithm.SSWWCv2, 1024, CO)
     _, _, prel, part_s = partitioned(sk, sp, fij.RadixPartitionAlgorithm.SSWWCv2, 1024, CO)
-    tasks = fij.DeviceArray.zeros(cus + 1, np.uint32)
-    ops["assign_tasks"] = line(timed(lambda: fij.assign_tasks(prel, bits, cus, tasks, stream=st)), 8 * (1 << bits))
+    jgrid = 2 * cus
+    tasks = fij.DeviceArray.zeros(jgrid + 1, np.uint32)
+    ops["assign_tasks"] = line(timed(lambda: fij.assign_tasks(prel, bits, jgrid, tasks, stream=st)), 8 * (1 << bits))
     want_build = int(sk.to_host().astype(np.int64).sum())
     joins = {}
     for scheme in (fij.HashingScheme.Perfect, fij.HashingScheme.BucketChaining):
-        j = fij.HipRadixJoin(FIRST, fij.RadixBits(bits), scheme, cus, 1024)
-        agg = fij.aggregates_array(cus)
+        j = fij.HipRadixJoin(FIRST, fij.RadixBits(bits), scheme, jgrid, 1024)
+        agg = fij.aggregates_array(jgrid)
         ms = timed(lambda: j.join(brel, prel, None, tasks, agg, stream=st), before=lambda: agg.fill(0, stream=st))
         got = fij.sum_aggregates(agg)
         assert got == (n_s, want_build, n_s * (n_s - 1) // 2), f"{scheme.name} join result {got}"
@@ -629,7 +632,7 @@ def operator_main(a):
         "config": {"workload": "operator API, one pass of 8 bits: R = permutation of 1..2^20 int32 (payload = key), "
                                "S = 2^24 uniform over R's keys; contiguous prefix sums + SSWWCv2 scatters + "
                                "chaining join",
-                   "partitioner_grid": cus, "join_grid": cus, "parallelism": "single GPU"},
+                   "partitioner_grid": cus, "join_grid": jgrid, "parallelism": "single GPU"},
         "ops": ops,
         "note": "bytes per op: prefix sums N k, copy with payload N 4k, scatters N (2k + t), joins (R + S) t, "
                 "assign_tasks 8 B per partition; frac = GB/s / 8000",
