@@ -32,8 +32,13 @@ struct JoinParams {
   uint32_t* status;
 };
 
+// Largest fanout whose task assignment runs by pointer doubling (its LDS:
+// prefix sums and two jump arrays of F + 1 entries).
+constexpr uint32_t kAssignDoublingMax = 8192;
+
 // radix_join.cu:58-98.  One block: partition sizes are staged in LDS in
-// parallel, then one thread runs the greedy cut exactly as the reference.
+// parallel, then the reference's greedy cut is computed in parallel, with
+// the same result.
 __global__ __launch_bounds__(1024) void assign_tasks_kernel(const uint64_t* probe_offsets,
                                                             uint32_t probe_len, uint32_t pad,
                                                             uint32_t radix_bits, uint32_t grid,
@@ -45,10 +50,50 @@ __global__ __launch_bounds__(1024) void assign_tasks_kernel(const uint64_t* prob
     sizes[p] = static_cast<uint32_t>(upper - probe_offsets[p]);
   }
   __syncthreads();
-  // The greedy rule (a task ends at the first partition whose size brings
-  // the task to at least avg probe tuples) taken 64 partitions at a time by
-  // wave 0: an inclusive scan of the sizes gives every lane the running sum
-  // since the last cut, a ballot finds the next cut; one step per cut.
+  // The greedy rule: a task ends at the first partition whose size brings
+  // the task to at least avg probe tuples.
+  if (F <= kAssignDoublingMax) {
+    // With P the exclusive prefix sums of the sizes, the task starting at
+    // partition s ends before J(s) = the first j > s with P[j] >= P[s] + avg
+    // (F if none), so ta[k] = J^k(0): J by binary searches, then J^m
+    // squared and applied to ta[0 .. m) in log2(grid) rounds.
+    __shared__ uint32_t wave_tmp[1024 / kWave + 1];
+    const uint32_t avg = (probe_len - pad * F + grid - 1u) / grid;
+    uint32_t* P = sizes;                                       // [F + 1]
+    uint16_t* cur = reinterpret_cast<uint16_t*>(sizes + F + 1);  // [F + 1]
+    uint16_t* nxt = cur + F + 1;                               // [F + 1]
+    const uint32_t total = block_exclusive_scan<uint32_t>(P, F, wave_tmp);
+    if (threadIdx.x == 0) P[F] = total;
+    __syncthreads();
+    for (uint32_t s0 = threadIdx.x; s0 <= F; s0 += blockDim.x) {
+      uint32_t lo = s0 + 1, hi = F + 1;
+      if (s0 < F) {
+        const uint64_t target = static_cast<uint64_t>(P[s0]) + avg;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (P[mid] >= target) hi = mid; else lo = mid + 1;
+        }
+      }
+      cur[s0] = static_cast<uint16_t>(min(lo, F));
+    }
+    if (threadIdx.x == 0) {
+      ta[0] = 0;
+      ta[grid] = F;
+    }
+    __syncthreads();
+    for (uint64_t m = 1; m < grid; m *= 2) {
+      for (uint64_t i = threadIdx.x; i < m && m + i < grid; i += blockDim.x) ta[m + i] = cur[ta[i]];
+      for (uint32_t s0 = threadIdx.x; s0 <= F; s0 += blockDim.x) nxt[s0] = cur[cur[s0]];
+      __syncthreads();
+      uint16_t* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    return;
+  }
+  // Larger fanouts: taken 64 partitions at a time by wave 0: an inclusive
+  // scan of the sizes gives every lane the running sum since the last cut,
+  // a ballot finds the next cut; one step per cut.
   if (threadIdx.x < kWave) {
     const uint32_t lane = threadIdx.x;
     const uint32_t probe_tuples = probe_len - pad * F;
@@ -275,7 +320,8 @@ hipError_t launch_assign_tasks(const uint64_t* probe_offsets, uint64_t probe_len
                                uint32_t radix_bits, uint32_t grid, uint32_t* task_assignments,
                                hipStream_t stream) {
   const uint32_t F = 1u << radix_bits;
-  hipLaunchKernelGGL(assign_tasks_kernel, dim3(1), dim3(1024), F * 4, stream, probe_offsets,
+  const size_t lds = F <= kAssignDoublingMax ? (F + 1) * 4 + 2 * (F + 1) * 2 : F * 4;
+  hipLaunchKernelGGL(assign_tasks_kernel, dim3(1), dim3(1024), lds, stream, probe_offsets,
                      static_cast<uint32_t>(probe_len), pad, radix_bits, grid, task_assignments);
   return hipGetLastError();
 }

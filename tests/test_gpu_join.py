@@ -159,3 +159,21 @@ def test_np_linear_probing_duplicates(dtype):
     got = fij.sum_aggregates(agg)
     assert got == orc.sort_merge_join(rk, rp, sk, sp)
     assert got == orc.np_join_lp(rk, rp, sk, sp, entries=entries)
+
+
+@pytest.mark.parametrize("bits,grid", [(13, 300), (13, 1), (14, 1000)])
+def test_task_assignment_wide_fanout(bits, grid):
+    """8192 partitions (the largest fanout assigned by pointer doubling) and
+    16384 (assigned by one wave, 64 partitions at a time): the same tasks as
+    the oracle's greedy rule."""
+    kt = fij.KeyType.I32
+    n = 1 << 20
+    sk = orc.gen_uniform(n, 5, 1, 0, 1 << 24, np.int32)
+    sp = orc.gen_iota(n, 0, np.int32)
+    prel = partition_one_pass(sk, sp, bits, kt)
+    tasks = fij.DeviceArray.zeros(grid + 1, np.uint32)
+    fij.assign_tasks(prel, bits, grid, tasks)
+    sync()
+    assert fij.device_status() == 0
+    ref = orc.assign_tasks(prel.host_offsets(), prel.padded_len(), orc.pad_len(np.int32), bits, grid)
+    np.testing.assert_array_equal(tasks.to_host(), ref)
