@@ -41,9 +41,9 @@ def parse():
     ap.add_argument("--chunks", type=int, default=0, help="pass-1 chunks (0 = automatic)")
     ap.add_argument("--block", type=int, default=0, help="pass-1 threads per block (0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-log2n", type=int, default=23, help="CPU baseline sample size (log2 of |R| = |S|)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--verify", action="store_true", help="check the result against the oracle at full size")
+    ap.add_argument("--cpu-log2n", type=int, default=0,
+                    help="CPU baseline size (log2 of |R| = |S|; 0 = the GPU workload's size)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = the host share)")
     ap.add_argument("--workload", choices=["plain", "skew", "batched", "served", "ooc", "np", "operator"],
                     default="plain",
                     help="plain: R = S = 2^27 int64, 9+9 bits (the headline line); skew: R = 2^27 int32 unique, "
@@ -69,9 +69,34 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# Host threads a job on one GPU of the box may use: the box gives each GPU a
+# share of 16 cores, whatever nproc reports for the whole machine.
+HOST_CPU_SHARE = 16
+
+
+def host_cpu():
+    """(cores this process may run on, CPU model name)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
 def cpu_baseline(log2n, bits1, bits2, threads):
     """The oracle's multi-threaded radix join (C restatement of the reference
-    CPU partitioner + per-partition hash join) on a bounded sample."""
+    CPU partitioner + per-partition hash join) on the same workload, built by
+    the same seeded generators as the device inputs.  Returns the baseline
+    line and the oracle's result for these inputs."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as orc
 
@@ -80,22 +105,28 @@ def cpu_baseline(log2n, bits1, bits2, threads):
     rp = rk.copy()
     sk = orc.gen_uniform(n, SEED, 1, 1, n, np.int64)
     sp = orc.gen_iota(n, 0, np.int64)
+    log(f"cpu baseline: inputs of 2 x 2^{log2n} tuples generated, {threads} threads")
     orc.radix_join(rk, rp, sk, sp, bits1, bits2, threads)  # warm-up
     times = []
-    t_end = time.time() + 15.0
+    t_end = time.time() + 20.0
     while len(times) < 3 or (time.time() < t_end and len(times) < 10):
         t0 = time.perf_counter()
         res = orc.radix_join(rk, rp, sk, sp, bits1, bits2, threads)
         times.append(time.perf_counter() - t0)
-    assert res[0] == n
+    log(f"cpu baseline: {len(times)} runs, median {np.median(times):.3f} s")
     med = float(np.median(times))
+    nproc, model = host_cpu()
     return {
         "value": 2 * n / med,
         "unit": "tuples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"R = S = 2^{log2n} int64 (same generators), {bits1}+{bits2} bits, median of {len(times)} runs",
-    }
+        "host_nproc": nproc,
+        "cpu_model": model,
+        "sample": f"R = S = 2^{log2n} int64 (the same seeded generators as the GPU inputs), {bits1}+{bits2} "
+                  f"bits, {threads} threads (one per core of the GPU's host share), 1 warm-up, median of "
+                  f"{len(times)} runs",
+    }, tuple(res)
 
 
 def measured_traffic(kernel, log2n, bits1, bits2):
@@ -187,10 +218,13 @@ def batched_main(a):
     2^10..2^22 with unique permuted keys, |S| = |R| x U{4..16} with keys
     uniform over that join's R (payloads: row indices).  The stream is
     generated and joined in waves of at most --wave-gib of input.  Measured:
-    the batched form (one submission per wave), and the plain per-join launch
-    (a plan created, executed and destroyed per join, as one run of the
-    reference's radix-join binary per pair) and the same with plans reused
-    per size class (one synchronous join at a time)."""
+    the batched form (one submission per wave) against the plain per-join
+    launch: one join at a time through a plan of its size class, plans
+    allocated beforehand and reused, so that no allocation is timed (the
+    reference's throughput excludes allocation, guides/benchmarking.md:45-56).
+    Reported beside it: the same launch with a plan created and destroyed per
+    join (one run of the reference's radix-join binary per pair), sampled on
+    every --plain-sample-th join."""
     import fij
 
     rng = np.random.default_rng(SEED)
@@ -208,8 +242,23 @@ def batched_main(a):
     waves.append(cur)
     batch = fij.JoinBatch(np.int32, max(len(w) for w in waves), use_graph=True)
     res_dev = fij.DeviceArray(3, np.uint64)
+    # per-join algorithmic bytes: a small join (one block, LDS table) reads
+    # each tuple's key and payload once (t); a partitioned join with P passes
+    # moves k + (2P + 1) t per tuple (key histogram, P scatters, join read)
+    classes = [fij.autotune(np.int32, int(r), int(s_)) for r, s_ in zip(n_r, n_s)]
+    b_alg = sum((int(r) + int(s_)) * (8 if small else 4 + (2 * (2 if b2 else 1) + 1) * 8)
+                for (small, _, b2), r, s_ in zip(classes, n_r, n_s))
+    # the plain per-join launch: one plan per size class, sized for the
+    # class's largest join, allocated before any timing
     pooled = {}
-    t_batch = t_plain = t_reused = 0.0
+    for (small, b1, b2), r, s_ in zip(classes, n_r, n_s):
+        cap = pooled.get((b1, b2), (0, 0))
+        pooled[(b1, b2)] = (max(cap[0], int(r)), max(cap[1], int(s_)))
+    pooled = {key: fij.JoinPlan(fij.KeyType.I32, r, s_, *key) for key, (r, s_) in pooled.items()}
+    plain_step = a.plain_sample or 16
+    t_alloc = 0.0
+    n_alloc = 0
+    t_batch = t_reused = 0.0
     gpu_batch = 0.0
     stats_acc = {"small_joins": 0, "partitioned_joins": 0, "prepare_ms": 0.0}
     checked = 0
@@ -239,28 +288,29 @@ def batched_main(a):
             stats_acc[k] += st[k]
         for (r, s_), g in zip(lens, got):
             assert g[0] == s_ and g[2] == s_ * (s_ - 1) // 2, "batched join result"
-        # the plain per-join launch (every k-th join when sampling)
-        step = a.plain_sample or 1
-        for j in range(0, len(joins), step):
+        # the plain per-join launch, one join at a time, every join
+        for j in range(len(joins)):
+            _, b1, b2 = classes[w[j]]
+            fij.synchronize()
+            t0 = time.perf_counter()
+            pooled[(b1, b2)].enqueue_sized(*joins[j], res_dev)
+            fij.synchronize()
+            t_reused += time.perf_counter() - t0
+            g = tuple(int(x) for x in res_dev.to_host())
+            assert g == got[j], "plain and batched results differ"
+            checked += 1
+        # with a plan allocated per join, on a sample
+        for j in range(0, len(joins), plain_step):
             r, s_ = lens[j]
-            small, b1, b2 = fij.autotune(np.int32, r, s_)
+            _, b1, b2 = classes[w[j]]
             t0 = time.perf_counter()
             plan = fij.JoinPlan(fij.KeyType.I32, r, s_, b1, b2)
             g, _ = plan.execute(*joins[j])
             del plan
             fij.synchronize()
-            t_plain += (time.perf_counter() - t0) * step
-            assert g == got[j], "plain and batched results differ"
-            key = (b1, b2)
-            cap = pooled.get(key, (None, 0, 0))
-            if cap[1] < r or cap[2] < s_:  # a plan per size class, grown to the largest join seen
-                pooled[key] = (fij.JoinPlan(fij.KeyType.I32, max(r, cap[1]), max(s_, cap[2]), b1, b2),
-                               max(r, cap[1]), max(s_, cap[2]))
-            t0 = time.perf_counter()
-            pooled[key][0].enqueue_sized(*joins[j], res_dev)
-            fij.synchronize()
-            t_reused += (time.perf_counter() - t0) * step
-            checked += 1
+            t_alloc += time.perf_counter() - t0
+            n_alloc += 1
+            assert g == got[j], "allocating plain launch and batched results differ"
         del joins, arena
         log(f"wave of {len(w)} joins done ({checked} checked against the plain launch)")
     value = total / t_batch
@@ -272,11 +322,16 @@ def batched_main(a):
                                "|S| = |R| x U{4..16}, count + sums per join", "joins": a.joins,
                    "tuples": total, "waves": len(waves), "parallelism": "single GPU"},
         "batched": {"seconds": t_batch, "gpu_seconds": gpu_batch, "tuples_per_s": value, **stats_acc},
-        "plain_per_join": {"seconds": t_plain, "tuples_per_s": total / t_plain,
-                           "sampled_every": a.plain_sample or 1},
-        "plain_reused_plans": {"seconds": t_reused, "tuples_per_s": total / t_reused},
-        "speedup_vs_plain": t_plain / t_batch,
-        "speedup_vs_reused_plans": t_reused / t_batch,
+        "algorithmic_bytes": b_alg,
+        "pipeline_roofline_frac": b_alg / gpu_batch / (HBM_PEAK_GBS * 1e9),
+        "wall_roofline_frac": b_alg / t_batch / (HBM_PEAK_GBS * 1e9),
+        "plain_per_join": {"seconds": t_reused, "tuples_per_s": total / t_reused,
+                           "note": "one join at a time through a preallocated plan of its size class"},
+        "speedup_vs_plain": t_reused / t_batch,
+        "plain_per_join_allocating": {
+            "seconds_per_join": t_alloc / max(n_alloc, 1), "joins_timed": n_alloc,
+            "note": "a plan created, executed and destroyed per join (allocation timed), every "
+                    f"{plain_step}th join"},
     }
     print(json.dumps(out))
 
@@ -290,9 +345,10 @@ def served_main(a):
     held in device memory.  Slots and resident builds are charged against
     the HBM budget.  Measured: capacity (all requests queued at once), then
     latency percentiles and throughput at a.load x capacity, and the plain
-    per-join launch (a plan created, executed and destroyed per join, both
-    sides partitioned, as one run of the reference's radix-join binary) over
-    the same stream.  Every result equals the same request joined alone."""
+    per-join launch over the same stream (both sides partitioned and joined,
+    one request at a time, through preallocated plans; beside it with a plan
+    allocated per request).  Every result equals the same request joined
+    alone."""
     import threading
 
     import fij
@@ -325,8 +381,18 @@ def served_main(a):
         probes.append((b, sk, sp))
     fij.synchronize()
     used, _ = srv.memory()
-    # each probe input joined alone, and by the plain per-join launch
-    alone, t_plain = [], []
+    # each probe input joined alone, and by the plain per-join launch: both
+    # sides partitioned and joined through a plan of the pair's size class,
+    # allocated before timing (no allocation timed); beside it, with a plan
+    # created and destroyed per join
+    alone, t_plain, t_alloc = [], [], []
+    classes = [fij.autotune(np.int64, int(n_b[b]), int(n_p[j])) for j, (b, _, _) in enumerate(probes)]
+    caps = {}
+    for (_, b1, b2), (b, _, _), j in zip(classes, probes, range(len(probes))):
+        c = caps.get((b1, b2), (0, 0))
+        caps[(b1, b2)] = (max(c[0], int(n_b[b])), max(c[1], int(n_p[j])))
+    pooled = {key: fij.JoinPlan(fij.KeyType.I64, r, s_, *key) for key, (r, s_) in caps.items()}
+    res_dev = fij.DeviceArray(3, np.uint64)
     bk = fij.DeviceArray(int(n_b.max()), np.int64)
     for j, (b, sk, sp) in enumerate(probes):
         alone.append(srv.wait(srv.submit(ids[b], sk, sp))[0])
@@ -334,15 +400,21 @@ def served_main(a):
         k, p = bk.view(0, n), bk.view(0, n)  # payload = key
         fij.gen_permutation(k, SEED + b, 1)
         fij.synchronize()
-        _, b1, b2 = fij.autotune(np.int64, n, int(n_p[j]))
+        _, b1, b2 = classes[j]
+        t0 = time.perf_counter()
+        pooled[(b1, b2)].enqueue_sized(k, p, sk, sp, res_dev)
+        fij.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+        g = tuple(int(x) for x in res_dev.to_host())
+        assert g == alone[j], f"probe input {j}: served {alone[j]} != plain {g}"
         t0 = time.perf_counter()
         plan = fij.JoinPlan(fij.KeyType.I64, n, int(n_p[j]), b1, b2)
         g, _ = plan.execute(k, p, sk, sp)
         del plan
         fij.synchronize()
-        t_plain.append(time.perf_counter() - t0)
-        assert g == alone[j], f"probe input {j}: served {alone[j]} != plain {g}"
-    del bk
+        t_alloc.append(time.perf_counter() - t0)
+        assert g == alone[j], f"probe input {j}: served {alone[j]} != allocating plain {g}"
+    del bk, pooled
     log(f"{a.probe_inputs} probe inputs joined alone and by the plain launch")
     stream = rng.integers(0, a.probe_inputs, a.requests)
     tuples = np.array([int(n_b[probes[j][0]]) + int(n_p[j]) for j in range(a.probe_inputs)], dtype=np.float64)
@@ -393,6 +465,7 @@ def served_main(a):
     # (k + 5t per tuple) plus the join's read of the resident build side (t)
     b_alg = float(sum(int(n_p[j]) * 88 + int(n_b[probes[j][0]]) * 16 for j in stream))
     plain_s = float(np.array(t_plain)[stream].sum())
+    alloc_s = float(np.array(t_alloc)[stream].sum())
     value = total / wall
     out = {
         "metric": METRIC, "value": value, "unit": "tuples/s", "n_gpus": 1, "steps": len(stream), "warmup": 1,
@@ -412,9 +485,13 @@ def served_main(a):
                        "max": float(lat.max())},
         "saturated": {"tuples_per_s": total / wall_sat, "p50_ms": float(np.percentile(lat_sat, 50)),
                       "p99_ms": float(np.percentile(lat_sat, 99))},
-        "plain_per_join": {"seconds": plain_s, "tuples_per_s": total / plain_s},
+        "plain_per_join": {"seconds": plain_s, "tuples_per_s": total / plain_s,
+                           "note": "each request's pair joined alone, both sides partitioned, through a "
+                                   "preallocated plan of its size class"},
         "speedup_vs_plain": (total / wall) / (total / plain_s),
         "speedup_vs_plain_saturated": plain_s / wall_sat,
+        "plain_per_join_allocating": {"seconds": alloc_s, "tuples_per_s": total / alloc_s,
+                                      "note": "a plan created, executed and destroyed per request"},
         "registration_s": t_register, "server_device_bytes": used,
     }
     print(json.dumps(out))
@@ -687,8 +764,11 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / a.steps
     gpu_ms = float(np.median(phases["total_ms"]))
-    # expected aggregate: every S key matches exactly one R key (payload = key)
-    assert res[0] == n, f"match count {res[0]} != {n}"
+    # closed forms: every S key matches exactly one R key, whose payload is
+    # the key, so SUM(R.value) = SUM(S.key); S's payload is the row index
+    sum_sk = int(sk.to_host().view(np.uint64).sum(dtype=np.uint64))
+    want = (n, sum_sk, (n * (n - 1) // 2) % (1 << 64))
+    assert tuple(res) == want, f"join result {tuple(res)} != closed form {want}"
 
     value = 2 * n / (ms_per_step / 1000.0)
     t_bytes = 16
@@ -747,16 +827,16 @@ def main():
         },
         "result": {"count": res[0], "sum_build": res[1], "sum_probe": res[2]},
     }
-    if a.verify:
-        sys.path.insert(0, os.path.join(REPO, "tests"))
-        import oracle_lib as orc
-
-        want = orc.radix_join(rk.to_host(), rp.to_host(), sk.to_host(), sp.to_host(), a.bits1, a.bits2,
-                              threads=os.cpu_count() or 8)
-        out["verified"] = tuple(res) == tuple(want)
+    out["result_check"] = "count, SUM(R.value) and SUM(S.value) equal their closed forms"
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(a.cpu_log2n, a.bits1, a.bits2, threads)
+        del rk, rp, sk, sp, plan
+        threads = a.cpu_threads or min(HOST_CPU_SHARE, host_cpu()[0])
+        cpu_log2n = a.cpu_log2n or a.log2n
+        out["cpu_baseline"], cpu_res = cpu_baseline(cpu_log2n, a.bits1, a.bits2, threads)
+        if cpu_log2n == a.log2n:
+            # the oracle joined the same inputs: its result must be the GPU's
+            assert cpu_res == tuple(res), f"oracle {cpu_res} != GPU {tuple(res)}"
+            out["cpu_baseline"]["same_result_as_gpu"] = True
     print(json.dumps(out))
 
 

@@ -134,6 +134,15 @@ __device__ __forceinline__ uint32_t perfect_index(K key, uint32_t ignore_bits) {
   return static_cast<uint32_t>(static_cast<uint64_t>(static_cast<int64_t>(key)) >> ignore_bits);
 }
 
+// Bucket of the chaining table: the key's index above the ignored bits (a
+// u32, as key_to_partition returns it) widened to the key type, then the
+// key type's multiply-shift hash -- 64-bit for i64 keys
+// (radix_join.cu:283-284,296-297: hash<K>(key_to_partition(...))).
+template <typename K>
+__device__ __forceinline__ uint32_t chain_bucket(K key, uint32_t ignore_bits, uint32_t log2_buckets) {
+  return mult_shift_hash<K>(static_cast<K>(perfect_index<K>(key, ignore_bits)), log2_buckets);
+}
+
 // Block-wide sum of three u64 values into aggregates[blockIdx.x] (+=).
 __device__ void block_aggregate(JoinAggregate* out, uint64_t c, uint64_t sb, uint64_t sp) {
   __shared__ uint64_t red[3][16];
@@ -245,7 +254,7 @@ __global__ __launch_bounds__(1024) void join_chaining_kernel(JoinParams p) {
       keys[i] = t.key;
       vals[i] = t.value;
       if (t.key == null_key<K>()) continue;  // stored but never linked: null keys never join
-      const uint32_t bucket = mult_shift_hash32(perfect_index<K>(t.key, p.ignore_bits), log2_buckets);
+      const uint32_t bucket = chain_bucket<K>(t.key, p.ignore_bits, log2_buckets);
       links[i] = static_cast<uint16_t>(atomicExch(&heads[bucket], i));
     }
     __syncthreads();
@@ -257,7 +266,7 @@ __global__ __launch_bounds__(1024) void join_chaining_kernel(JoinParams p) {
 #pragma unroll
       for (uint32_t k = 0; k < kJoinLoads; ++k) {
         if (i0 + k * blockDim.x + threadIdx.x >= q1 || t[k].key == null_key<K>()) continue;
-        const uint32_t bucket = mult_shift_hash32(perfect_index<K>(t[k].key, p.ignore_bits), log2_buckets);
+        const uint32_t bucket = chain_bucket<K>(t[k].key, p.ignore_bits, log2_buckets);
         for (uint32_t e = heads[bucket]; e != kChainTail; e = links[e]) {
           if (keys[e] == t[k].key) {
             cnt += 1;

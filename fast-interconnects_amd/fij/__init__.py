@@ -1082,7 +1082,9 @@ _proto("fij_ooc_join", C.c_int, C.POINTER(OocConfig), _vp, _vp, _u64, _vp, _vp, 
        C.POINTER(OocTiming))
 
 
-def _col_ptr(x):
+def _col_ptr(x, dtype):
+    if x.dtype != dtype:
+        raise ValueError(f"column of {x.dtype} in a join of {dtype}")
     if isinstance(x, np.ndarray):
         if not x.flags.c_contiguous:
             raise ValueError("columns must be contiguous")
@@ -1096,7 +1098,10 @@ def ooc_join(dtype, bk, bp, sk, sp, bits1, bits2, pass1_cpu=False, cache_bytes=0
     HostArray or DeviceArray.  Returns ((count, sum_build, sum_probe), timing)."""
     cfg = OocConfig(int(_ktype(dtype)), bits1, bits2, 1 if pass1_cpu else 0, cpu_threads, 0, cache_bytes,
                     staging_tuples)
-    (bkp, nb), (bpp, _), (skp, ns), (spp, _) = (_col_ptr(x) for x in (bk, bp, sk, sp))
+    want = numpy_dtype(_ktype(dtype))
+    (bkp, nb), (bpp, nbp), (skp, ns), (spp, nsp) = (_col_ptr(x, want) for x in (bk, bp, sk, sp))
+    if nbp != nb or nsp != ns:
+        raise ValueError("Join and payload attributes have different sizes")
     res = JoinAggregate()
     t = OocTiming()
     check(lib.fij_ooc_join(C.byref(cfg), bkp, bpp, nb, skp, spp, ns, C.byref(res), C.byref(t)))

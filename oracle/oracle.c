@@ -337,15 +337,23 @@ int64_t orc_mult_shift_hash_i64(int64_t value, uint32_t log2_buckets) {
   uint64_t product = (uint64_t)value * 11400714819323198485ULL;
   return (int64_t)(product >> (64u - log2_buckets));
 }
-int32_t orc_murmur3_hash_i32(int32_t value, uint32_t log2_buckets) {
-  uint32_t h = (uint32_t)value;
+/* MurmurHash3's finalizers fmix32 / fmix64 (Appleby's public-domain
+ * MurmurHash3.cpp), which gpu_common.h:181-212 applies to the key and then
+ * masks to log2_buckets bits.  Exported whole so that tests can run the full
+ * MurmurHash3_x86_32 / x64_128 on top of them against published vectors. */
+uint32_t orc_fmix32(uint32_t h) {
   h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
-  return (int32_t)(h & ((1u << log2_buckets) - 1u));
+  return h;
+}
+uint64_t orc_fmix64(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+  return k;
+}
+int32_t orc_murmur3_hash_i32(int32_t value, uint32_t log2_buckets) {
+  return (int32_t)(orc_fmix32((uint32_t)value) & ((1u << log2_buckets) - 1u));
 }
 int64_t orc_murmur3_hash_i64(int64_t value, uint32_t log2_buckets) {
-  uint64_t k = (uint64_t)value;
-  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
-  return (int64_t)(k & ((1ULL << log2_buckets) - 1ULL));
+  return (int64_t)(orc_fmix64((uint64_t)value) & ((1ULL << log2_buckets) - 1ULL));
 }
 
 static uint64_t next_pow2(uint64_t x) {

@@ -104,6 +104,8 @@ void orc_assign_tasks(const uint64_t *probe_offsets, uint32_t probe_len,
                       uint32_t *task_assignments /* [grid + 1] */);
 int32_t orc_mult_shift_hash_i32(int32_t value, uint32_t log2_buckets);
 int64_t orc_mult_shift_hash_i64(int64_t value, uint32_t log2_buckets);
+uint32_t orc_fmix32(uint32_t h);
+uint64_t orc_fmix64(uint64_t k);
 int32_t orc_murmur3_hash_i32(int32_t value, uint32_t log2_buckets);
 int64_t orc_murmur3_hash_i64(int64_t value, uint32_t log2_buckets);
 

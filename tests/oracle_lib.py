@@ -65,6 +65,8 @@ _proto("orc_partition_len", _u64, _vp, _u32, _u32, _u64, _u32, _u32)
 _proto("orc_assign_tasks", None, _vp, _u32, _u32, _u32, _u32, _vp)
 _proto("orc_mult_shift_hash_i32", _i32, _i32, _u32)
 _proto("orc_mult_shift_hash_i64", _i64, _i64, _u32)
+_proto("orc_fmix32", _u32, _u32)
+_proto("orc_fmix64", _u64, _u64)
 _proto("orc_murmur3_hash_i32", _i32, _i32, _u32)
 _proto("orc_murmur3_hash_i64", _i64, _i64, _u32)
 

@@ -318,5 +318,21 @@ def test_ooc_arguments_rejected_on_host():
         with pytest.raises(fij.FijError) as e:
             fij.ooc_join(np.int64, k, k, k, k, *bits, pass1_cpu=True)
         assert e.value.status == fij.Status.INVALID_ARGUMENT
+    raw = np.zeros(8 * 1000 + 8, np.uint8)
+    misaligned = np.frombuffer(raw.data, np.int64, 999, offset=4)
     with pytest.raises(fij.FijError):
-        fij.ooc_join(np.int64, k[1:].view(np.int32)[1:1000], k[:999], k, k, 4, 4, pass1_cpu=True)  # misaligned
+        fij.ooc_join(np.int64, misaligned, k[:999], k, k, 4, 4, pass1_cpu=True)
+
+
+def test_ooc_columns_checked_before_the_call():
+    """Payload columns shorter than their keys, or columns of another type,
+    would make the C call read past their ends: rejected in Python."""
+    k = np.arange(1, 1001, dtype=np.int64)
+    with pytest.raises(ValueError):
+        fij.ooc_join(np.int64, k, k[:999], k, k, 4, 4, pass1_cpu=True)
+    with pytest.raises(ValueError):
+        fij.ooc_join(np.int64, k, k, k, k[:10], 4, 4, pass1_cpu=True)
+    with pytest.raises(ValueError):
+        fij.ooc_join(np.int64, k.astype(np.int32), k, k, k, 4, 4, pass1_cpu=True)
+    with pytest.raises(ValueError):
+        fij.ooc_join(np.int32, k, k, k, k, 4, 4, pass1_cpu=True)

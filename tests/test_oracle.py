@@ -196,11 +196,106 @@ def test_mult_shift_hash_known_answers():
     assert orc.o.orc_mult_shift_hash_i32(0, 11) == 0
 
 
-def test_murmur3_finalizer_known_answers():
-    # fmix32/fmix64 fix zero and mix a single bit into many
-    assert orc.o.orc_murmur3_hash_i32(0, 16) == 0
-    assert orc.o.orc_murmur3_hash_i64(0, 16) == 0
-    assert bin(orc.o.orc_murmur3_hash_i32(1, 31) & 0xFFFFFFFF).count("1") > 8
+# MurmurHash3 (Appleby, SMHasher) built on the oracle's finalizers.  The body
+# below is the published hash; only fmix32/fmix64 come from the oracle, so the
+# published outputs of the whole hash pin both finalizers: a wrong constant or
+# shift in either changes every vector.
+_M32, _M64 = 0xFFFFFFFF, (1 << 64) - 1
+
+
+def _rotl(x, r, bits):
+    m = (1 << bits) - 1
+    return ((x << r) | (x >> (bits - r))) & m
+
+
+def _murmur3_x86_32(data, seed):
+    c1, c2 = 0xCC9E2D51, 0x1B873593
+    h = seed & _M32
+    nb = len(data) // 4
+    for i in range(nb):
+        k = int.from_bytes(data[4 * i:4 * i + 4], "little")
+        k = (_rotl((k * c1) & _M32, 15, 32) * c2) & _M32
+        h = (_rotl(h ^ k, 13, 32) * 5 + 0xE6546B64) & _M32
+    tail = data[4 * nb:]
+    if tail:
+        k = int.from_bytes(tail, "little")
+        h ^= (_rotl((k * c1) & _M32, 15, 32) * c2) & _M32
+    return orc.o.orc_fmix32(h ^ len(data))
+
+
+def _murmur3_x64_128(data, seed):
+    c1, c2 = 0x87C37B91114253D5, 0x4CF5AD432745937F
+    h1 = h2 = seed & _M32
+    nb = len(data) // 16
+    for i in range(nb):
+        k1 = int.from_bytes(data[16 * i:16 * i + 8], "little")
+        k2 = int.from_bytes(data[16 * i + 8:16 * i + 16], "little")
+        h1 ^= (_rotl((k1 * c1) & _M64, 31, 64) * c2) & _M64
+        h1 = ((_rotl(h1, 27, 64) + h2) * 5 + 0x52DCE729) & _M64
+        h2 ^= (_rotl((k2 * c2) & _M64, 33, 64) * c1) & _M64
+        h2 = ((_rotl(h2, 31, 64) + h1) * 5 + 0x38495AB5) & _M64
+    tail = data[16 * nb:]
+    if len(tail) > 8:
+        k2 = int.from_bytes(tail[8:], "little")
+        h2 ^= (_rotl((k2 * c2) & _M64, 33, 64) * c1) & _M64
+    if tail:
+        k1 = int.from_bytes(tail[:8], "little")
+        h1 ^= (_rotl((k1 * c1) & _M64, 31, 64) * c2) & _M64
+    h1 ^= len(data)
+    h2 ^= len(data)
+    h1 = (h1 + h2) & _M64
+    h2 = (h2 + h1) & _M64
+    h1 = orc.o.orc_fmix64(h1)
+    h2 = orc.o.orc_fmix64(h2)
+    h1 = (h1 + h2) & _M64
+    h2 = (h2 + h1) & _M64
+    return h1.to_bytes(8, "little") + h2.to_bytes(8, "little")
+
+
+def _smhasher_verification(hash_bytes):
+    """SMHasher's VerificationTest: hash {}, {0}, {0,1}, ... {0..254} with seed
+    256 - len, hash the concatenated results with seed 0, and read the first
+    four bytes little-endian."""
+    key = bytes(range(256))
+    hashes = b"".join(hash_bytes(key[:i], 256 - i) for i in range(256))
+    return int.from_bytes(hash_bytes(hashes, 0)[:4], "little")
+
+
+def test_murmur3_x86_32_published_vectors():
+    # SMHasher's verification value for MurmurHash3_x86_32
+    assert _smhasher_verification(lambda d, s: _murmur3_x86_32(d, s).to_bytes(4, "little")) == 0xB0F57EE3
+    # widely published single-input outputs of MurmurHash3_x86_32
+    for data, seed, want in [
+        (b"", 0, 0),
+        (b"", 1, 0x514E28B7),
+        (b"", 0xFFFFFFFF, 0x81F16F39),
+        (b"\0\0\0\0", 0, 0x2362F9DE),
+        (b"aaaa", 0x9747B28C, 0x5A97808A),
+        (b"Hello, world!", 0x9747B28C, 0x24884CBA),
+        (b"The quick brown fox jumps over the lazy dog", 0x9747B28C, 0x2FA826CD),
+    ]:
+        assert _murmur3_x86_32(data, seed) == want, (data, seed)
+
+
+def test_murmur3_x64_128_published_vectors():
+    # SMHasher's verification value for MurmurHash3_x64_128
+    assert _smhasher_verification(_murmur3_x64_128) == 0x6384BA69
+    got = _murmur3_x64_128(b"The quick brown fox jumps over the lazy dog", 0)
+    assert got.hex() == "6c1b07bc7bbc4be347939ac4a93c437a"
+
+
+def test_murmur3_hash_is_masked_finalizer():
+    # gpu_common.h:181-212: fmix of the key, masked to log2_buckets bits; the
+    # finalizer itself is pinned by the published vectors above
+    for v in (0, 1, 7, -1, 123456789, -2**31):
+        for lg in (1, 11, 16, 31):
+            want = orc.o.orc_fmix32(v & _M32) & ((1 << lg) - 1)
+            assert orc.o.orc_murmur3_hash_i32(v, lg) == want
+    for v in (0, 1, -1, 2**40 + 5, -2**63):
+        for lg in (1, 11, 33, 63):
+            want = orc.o.orc_fmix64(v & _M64) & ((1 << lg) - 1)
+            assert orc.o.orc_murmur3_hash_i64(v, lg) == want
+    assert orc.o.orc_fmix32(1) == 0x514E28B7  # MurmurHash3_x86_32("", seed 1)
 
 
 # ---- a10/a11/a13/a14 joins ---------------------------------------------------
