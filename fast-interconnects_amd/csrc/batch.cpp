@@ -25,6 +25,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -80,6 +82,15 @@ struct ClassArena {
   uint32_t sub_joins = 0, sub_blocks = 0, sub_tiles[2] = {0, 0};
 };
 
+// A size class's configuration as batches run it: radix bits, pass-1 line
+// ring cap (0 = deepest) and pass-1 block shape.
+struct ClassKey {
+  uint32_t b1, b2, ring, block;
+  bool operator<(const ClassKey& o) const {
+    return std::tie(b1, b2, ring, block) < std::tie(o.b1, o.b2, o.ring, o.block);
+  }
+};
+
 // One join side's place in a class arena.
 struct SideLayout {
   uint64_t n;
@@ -101,7 +112,7 @@ struct fij_join_batch {
   JoinAggregate* h_results = nullptr;
   hipStream_t own = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
-  std::map<std::pair<uint32_t, uint32_t>, ClassArena> arenas;
+  std::map<ClassKey, ClassArena> arenas;
 };
 
 namespace {
@@ -200,8 +211,9 @@ fij_status grow_arena(ClassArena& a, KeyType kt, uint32_t b1, uint32_t b2, uint6
 }
 
 // Records one size class's passes over its joins (join indices `ids`).
-fij_status enqueue_class(fij_join_batch* b, ClassArena& a, uint32_t b1, uint32_t b2, const std::vector<uint32_t>& ids,
+fij_status enqueue_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, const std::vector<uint32_t>& ids,
                          const fij_join_desc* joins, hipStream_t st, uint32_t lds_total, std::vector<uint8_t>& host) {
+  const uint32_t b1 = ck.b1, b2 = ck.b2;
   const KeyType kt = b->kt;
   const fij_type type = static_cast<fij_type>(b->cfg.type);
   const uint32_t F1 = 1u << b1, F2 = 1u << b2;
@@ -257,12 +269,12 @@ fij_status enqueue_class(fij_join_batch* b, ClassArena& a, uint32_t b1, uint32_t
       PipelineSide& ps = side ? a.d.probe : a.d.build;
       const void* keys = side ? jd.probe_keys : jd.build_keys;
       const void* pays = side ? jd.probe_payloads : jd.build_payloads;
-      hh[js] = hist_params(kt, keys, l.n, l.chunks, b1, 1024, a.counts + l.counts_at);
+      hh[js] = hist_params(kt, keys, l.n, l.chunks, b1, ck.block, a.counts + l.counts_at);
       hs[js] = BatchScanJob{a.counts + l.counts_at, a.local + l.local_at, ps.off1 + static_cast<uint64_t>(j) * F1,
                             l.rel1_at, F1, l.chunks, pad};
       void* region = static_cast<uint8_t*>(ps.rel1) + l.rel1_at * tb;
       const hipError_t e = scatter_params_batched(kt, keys, pays, l.n, l.chunks, b1, a.local + l.local_at, region,
-                                                  lds_total, &hc[js]);
+                                                  lds_total, ck.block, ck.ring, &hc[js]);
       if (e != hipSuccess) return batch_fail(FIJ_UNSUPPORTED, "size class %u+%u: pass-1 buffers do not fit LDS", b1, b2);
       hb[js] = BatchSide{l.rel1_at + l.n + static_cast<uint64_t>(F1) * pad, l.rel2_at, j * F1, 0};
       hp[js] = at;
@@ -302,13 +314,16 @@ fij_status enqueue_class(fij_join_batch* b, ClassArena& a, uint32_t b1, uint32_t
 
 // The class's launches (after enqueue_class prepared its descriptors): one
 // per pass over all its joins.
-hipError_t launch_class(fij_join_batch* b, ClassArena& a, uint32_t b1, hipStream_t st, uint32_t lds_total) {
+hipError_t launch_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, hipStream_t st, uint32_t lds_total) {
   const KeyType kt = b->kt;
+  const uint32_t b1 = ck.b1;
   const uint32_t js = 2 * a.sub_joins;
   hipError_t e;
-  if ((e = launch_hist_batched(kt, a.hist, a.prefix, js, a.sub_blocks, b1, 1024, st)) != hipSuccess) return e;
+  if ((e = launch_hist_batched(kt, a.hist, a.prefix, js, a.sub_blocks, b1, ck.block, st)) != hipSuccess) return e;
   if ((e = launch_offsets_batched(a.scan, js, st)) != hipSuccess) return e;
-  if ((e = launch_partition_batched(kt, a.scat, a.prefix, js, a.sub_blocks, b1, lds_total, st)) != hipSuccess) return e;
+  if ((e = launch_partition_batched(kt, a.scat, a.prefix, js, a.sub_blocks, b1, lds_total, ck.block, ck.ring, st)) !=
+      hipSuccess)
+    return e;
   if ((e = launch_pipeline_plan_batched(a.d, a.sides, a.sub_joins, a.side_tiles, a.d.build.n, a.d.probe.n, st)) !=
       hipSuccess)
     return e;
@@ -319,18 +334,14 @@ hipError_t launch_class(fij_join_batch* b, ClassArena& a, uint32_t b1, hipStream
   return launch_pipeline_join(a.d, kt, false, true, 2 * b->cus, st);
 }
 
-}  // namespace
-
-extern "C" {
-
-fij_status fij_join_autotune(fij_type type, uint64_t build_len, uint64_t probe_len, uint32_t* small,
-                             uint32_t* radix_bits_1, uint32_t* radix_bits_2) {
-  if ((type != FIJ_TYPE_I32 && type != FIJ_TYPE_I64) || !small || !radix_bits_1 || !radix_bits_2)
-    return batch_fail(FIJ_INVALID_ARGUMENT, "invalid argument");
-  const KeyType kt = type == FIJ_TYPE_I32 ? KeyType::I32 : KeyType::I64;
+// The closed-form size-class rule (the built-in table's fallback).
+fij_join_tuning class_rule(KeyType kt, uint64_t build_len, uint64_t probe_len) {
+  fij_join_tuning t{};
+  t.build_log2 = ceil_log2(build_len);
+  t.probe_log2 = ceil_log2(probe_len);
   // one block joins a small pair: its build side fits one LDS table fill and
   // its probe side is short enough for one block to stream
-  *small = build_len <= small_join_table_tuples(kt) && probe_len <= (1u << 22) ? 1u : 0u;
+  t.small = build_len <= small_join_table_tuples(kt) && probe_len <= (1u << 22) ? 1u : 0u;
   // Size classes of the radix plan: at most 512 build and 512 probe tuples
   // per (pass-1, pass-2) partition pair on average, so that pairs fit the
   // join's registers (at most 640 per side), split evenly over the passes
@@ -343,8 +354,98 @@ fij_status fij_join_autotune(fij_type type, uint64_t build_len, uint64_t probe_l
   const uint32_t need = ceil_log2(probe_len) > 19 ? ceil_log2(probe_len) - 19 : 0;
   b1 = std::min<uint32_t>(9, std::max(b1, need));
   b = std::max(b, b1 + 1);
-  *radix_bits_1 = b1;
-  *radix_bits_2 = std::min<uint32_t>(12, b - b1);
+  t.radix_bits_1 = b1;
+  t.radix_bits_2 = std::min<uint32_t>(12, b - b1);
+  t.ring_lines = 0;
+  t.block = 1024;
+  t.from_table = 0;
+  return t;
+}
+
+// Built-in tables: the configurations measured fastest per size class on
+// MI355X (tuning_table.inc, written by fast-interconnects_amd/autotune.py).
+// Each list ends with an entry of block 0.
+#include "tuning_table.inc"
+
+std::vector<fij_join_tuning> builtin_table(KeyType kt) {
+  std::vector<fij_join_tuning> out;
+  for (const fij_join_tuning* e = kt == KeyType::I32 ? kTunedI32 : kTunedI64; e->block; ++e) {
+    out.push_back(*e);
+    out.back().from_table = 1;
+  }
+  return out;
+}
+
+std::mutex g_tuning_mutex;
+std::vector<fij_join_tuning> g_tuning[2];
+bool g_tuning_loaded[2] = {false, false};
+
+// The configuration of a pair of these lengths: the table entry of its class
+// when there is one (the small path only where the pair qualifies for it),
+// otherwise the rule.
+fij_join_tuning tuned_class(KeyType kt, uint64_t build_len, uint64_t probe_len) {
+  const fij_join_tuning rule = class_rule(kt, build_len, probe_len);
+  const int ti = kt == KeyType::I32 ? 0 : 1;
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  if (!g_tuning_loaded[ti]) {
+    g_tuning[ti] = builtin_table(kt);
+    g_tuning_loaded[ti] = true;
+  }
+  for (const fij_join_tuning& e : g_tuning[ti]) {
+    if (e.build_log2 != rule.build_log2 || e.probe_log2 != rule.probe_log2) continue;
+    fij_join_tuning t = e;
+    t.small = e.small && rule.small;
+    t.from_table = 1;
+    return t;
+  }
+  return rule;
+}
+
+}  // namespace
+
+extern "C" {
+
+fij_status fij_join_autotune(fij_type type, uint64_t build_len, uint64_t probe_len, uint32_t* small,
+                             uint32_t* radix_bits_1, uint32_t* radix_bits_2) {
+  if (!small || !radix_bits_1 || !radix_bits_2) return batch_fail(FIJ_INVALID_ARGUMENT, "invalid argument");
+  fij_join_tuning t{};
+  const fij_status s = fij_join_tuning_lookup(type, build_len, probe_len, &t);
+  if (s != FIJ_OK) return s;
+  *small = t.small;
+  *radix_bits_1 = t.radix_bits_1;
+  *radix_bits_2 = t.radix_bits_2;
+  return FIJ_OK;
+}
+
+fij_status fij_join_tuning_lookup(fij_type type, uint64_t build_len, uint64_t probe_len, fij_join_tuning* out) {
+  if ((type != FIJ_TYPE_I32 && type != FIJ_TYPE_I64) || !out) return batch_fail(FIJ_INVALID_ARGUMENT, "invalid argument");
+  *out = tuned_class(type == FIJ_TYPE_I32 ? KeyType::I32 : KeyType::I64, build_len, probe_len);
+  return FIJ_OK;
+}
+
+fij_status fij_join_tuning_set(fij_type type, const fij_join_tuning* entries, uint32_t n, int use_builtin) {
+  if ((type != FIJ_TYPE_I32 && type != FIJ_TYPE_I64) || (n && !entries))
+    return batch_fail(FIJ_INVALID_ARGUMENT, "invalid argument");
+  std::vector<fij_join_tuning> table;
+  if (use_builtin) {
+    table = builtin_table(type == FIJ_TYPE_I32 ? KeyType::I32 : KeyType::I64);
+  } else {
+    for (uint32_t i = 0; i < n; ++i) {
+      const fij_join_tuning& e = entries[i];
+      const bool pow2_ring = e.ring_lines == 0 || (e.ring_lines <= 256 && (e.ring_lines & (e.ring_lines - 1)) == 0);
+      if (e.radix_bits_1 < 1 || e.radix_bits_1 > 9 || e.radix_bits_2 < 1 || e.radix_bits_2 > 12 || !pow2_ring ||
+          (e.block != 512 && e.block != 1024) || e.small > 1 || e.build_log2 > 40 || e.probe_log2 > 40)
+        return batch_fail(FIJ_INVALID_ARGUMENT,
+                          "tuning entry %u: radix bits %u+%u (1..9 + 1..12), ring %u (0 or a power of two <= 256), "
+                          "block %u (512 or 1024)",
+                          i, e.radix_bits_1, e.radix_bits_2, e.ring_lines, e.block);
+      table.push_back(e);
+      table.back().from_table = 1;
+    }
+  }
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  g_tuning[type == FIJ_TYPE_I32 ? 0 : 1] = std::move(table);
+  g_tuning_loaded[type == FIJ_TYPE_I32 ? 0 : 1] = true;
   return FIJ_OK;
 }
 
@@ -404,11 +505,10 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
   if (n > b->cfg.max_joins) return batch_fail(FIJ_INVALID_ARGUMENT, "more joins than the batch was created for");
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b->own;
-  const fij_type type = static_cast<fij_type>(b->cfg.type);
   const size_t kb = key_bytes(b->kt);
   // classify
   uint32_t n_small = 0, n_large = 0;
-  std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> classes;  // radix bits -> join indices
+  std::map<ClassKey, std::vector<uint32_t>> classes;  // configuration -> join indices
   for (uint32_t i = 0; i < n; ++i) {
     const fij_join_desc& j = joins[i];
     for (const void* c : {j.build_keys, j.build_payloads, j.probe_keys, j.probe_payloads})
@@ -416,10 +516,8 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
         return batch_fail(FIJ_INVALID_ARGUMENT, "join %u: columns must be aligned to their element size", i);
     if ((j.build_len && (!j.build_keys || !j.build_payloads)) || (j.probe_len && (!j.probe_keys || !j.probe_payloads)))
       return batch_fail(FIJ_INVALID_ARGUMENT, "join %u: NULL input column", i);
-    uint32_t small = 0, b1 = 0, b2 = 0;
-    fij_status s = fij_join_autotune(type, j.build_len, j.probe_len, &small, &b1, &b2);
-    if (s != FIJ_OK) return s;
-    if (small || j.probe_len == 0) {
+    const fij_join_tuning tc = tuned_class(b->kt, j.build_len, j.probe_len);
+    if (tc.small || j.probe_len == 0) {
       SmallJob& sj = b->h_jobs[n_small++];
       sj.build_keys = j.build_keys;
       sj.build_pays = j.build_payloads;
@@ -429,15 +527,14 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
       sj.probe_len = j.probe_len;
       sj.result = b->d_results + i;
     } else {
-      classes[{b1, b2}].push_back(i);
+      classes[ClassKey{tc.radix_bits_1, tc.radix_bits_2, tc.ring_lines, tc.block}].push_back(i);
       ++n_large;
     }
   }
   // every class's descriptors and (grown) buffers
   std::vector<uint8_t> host;
   for (auto& kv : classes) {
-    fij_status s = enqueue_class(b, b->arenas[kv.first], kv.first.first, kv.first.second, kv.second, joins, st,
-                                 b->lds_total, host);
+    fij_status s = enqueue_class(b, b->arenas[kv.first], kv.first, kv.second, joins, st, b->lds_total, host);
     if (s != FIJ_OK) return s;
   }
   if (n_small) {
@@ -457,9 +554,9 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
   }
   for (auto& kv : classes) {
     if (s != FIJ_OK) break;
-    hipError_t e = launch_class(b, b->arenas[kv.first], kv.first.first, st, b->lds_total);
-    if (e != hipSuccess) s = batch_fail(FIJ_DEVICE_ERROR, "size class %u+%u: %s", kv.first.first, kv.first.second,
-                                        hipGetErrorString(e));
+    hipError_t e = launch_class(b, b->arenas[kv.first], kv.first, st, b->lds_total);
+    if (e != hipSuccess)
+      s = batch_fail(FIJ_DEVICE_ERROR, "size class %u+%u: %s", kv.first.b1, kv.first.b2, hipGetErrorString(e));
   }
   if (use_graph) {
     hipError_t e = hipStreamEndCapture(st, &graph);
@@ -512,7 +609,7 @@ fij_status fij_join_batch_device_bytes(const fij_join_batch* b, uint64_t* bytes)
   const uint64_t tb = tuple_bytes(b->kt);
   for (const auto& kv : b->arenas) {
     const ClassArena& a = kv.second;
-    const uint64_t parts = a.joins << kv.first.first, F2 = 1ull << kv.first.second;
+    const uint64_t parts = a.joins << kv.first.b1, F2 = 1ull << kv.first.b2;
     for (int s = 0; s < 2; ++s)
       total += (a.rel1[s] + a.rel2[s]) * tb + parts * (8 + 4 + 8 + 4) + (a.tiles[s] + 1) * F2 * 4;
     total += a.chunk_words * (4 + 8) + parts * F2 + a.units * 16 +

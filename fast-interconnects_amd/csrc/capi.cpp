@@ -350,8 +350,15 @@ fij_status fij_partitioner_destroy(fij_partitioner* p) {
   return FIJ_OK;
 }
 
-fij_status fij_prefix_sum(fij_partitioner* p, fij_radix_pass pass, fij_type type, const void* partition_attr,
-                          uint64_t len, fij_partition_offsets* offsets, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Argument checks of prefix_sum (gpu_radix_partition.rs:561-696); sets *bits
+// to the pass's radix bits.
+fij_status prefix_sum_checks(const fij_partitioner* p, fij_radix_pass pass, fij_type type,
+                             const void* partition_attr, uint64_t len, const fij_partition_offsets* offsets,
+                             uint32_t* bits_out) {
   if (!p || !offsets || !valid_type(type) || (len && !partition_attr))
     return fail(FIJ_INVALID_ARGUMENT, "invalid argument");
   uint32_t bits = 0;
@@ -376,6 +383,19 @@ fij_status fij_prefix_sum(fij_partitioner* p, fij_radix_pass pass, fij_type type
   if (s != FIJ_OK) return s;
   if (partition_attr && !aligned_to(partition_attr, key_bytes(key_type(type))))
     return fail(FIJ_INVALID_ARGUMENT, "key column is not aligned to its element size");
+  *bits_out = bits;
+  return FIJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fij_status fij_prefix_sum(fij_partitioner* p, fij_radix_pass pass, fij_type type, const void* partition_attr,
+                          uint64_t len, fij_partition_offsets* offsets, void* stream) {
+  uint32_t bits = 0;
+  fij_status s = prefix_sum_checks(p, pass, type, partition_attr, len, offsets, &bits);
+  if (s != FIJ_OK) return s;
   DeviceInfo* d = nullptr;
   s = device_info(&d);
   if (s != FIJ_OK) return s;

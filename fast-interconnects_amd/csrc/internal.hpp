@@ -27,7 +27,7 @@ enum DeviceStatus : uint32_t {
 constexpr uint32_t kScanBlock = 1024;
 constexpr uint32_t kScanItems = 8;
 constexpr uint32_t kScanTile = kScanBlock * kScanItems;
-inline uint64_t scan_state_words(uint64_t n) { return 1 + (n + kScanTile - 1) / kScanTile; }
+constexpr uint64_t scan_state_words(uint64_t n) { return 1 + (n + kScanTile - 1) / kScanTile; }
 
 // ---- prefix sums (hist.hip) ----------------------------------------------
 // Chunked: one block per chunk; writes offsets[chunk][F] directly.
@@ -134,13 +134,18 @@ struct ScatterParams {
   uint32_t spill_from;   // first partition written to the spill region
 };
 
+// Scatter descriptor of one relation of a batched launch: the line ring is
+// the deepest that fits LDS for this block shape (512 or 1024 threads), at
+// most ring_cap lines per partition when ring_cap is set.
 hipError_t scatter_params_batched(KeyType t, const void* keys, const void* pays, uint64_t n, uint32_t chunks,
                                   uint32_t radix_bits, const uint64_t* chunk_offsets, void* relation,
-                                  uint32_t lds_total, ScatterParams* out);
+                                  uint32_t lds_total, uint32_t block, uint32_t ring_cap, ScatterParams* out);
 // Scatters of njobs relations of one fanout in one launch: block b scatters
-// chunk b - prefix[j] of jobs[j] (prefix[njobs] = blocks, device memory).
+// chunk b - prefix[j] of jobs[j] (prefix[njobs] = blocks, device memory);
+// block and ring_cap as given to scatter_params_batched for every job.
 hipError_t launch_partition_batched(KeyType t, const ScatterParams* jobs, const uint32_t* prefix, uint32_t njobs,
-                                    uint32_t blocks, uint32_t radix_bits, uint32_t lds_total, hipStream_t stream);
+                                    uint32_t blocks, uint32_t radix_bits, uint32_t lds_total, uint32_t block,
+                                    uint32_t ring_cap, hipStream_t stream);
 // Pass 1 on the CPU (host memory in and out): the contiguous layout of the
 // GPU's pass 1; offsets[F] receives the partition starts.
 void cpu_partition_contiguous(KeyType t, const void* keys, const void* pays, uint64_t n, uint32_t bits,

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fij.h"
@@ -189,22 +190,252 @@ extern "C" fij_status fij_ooc_join(const fij_ooc_config* cfg, const void* build_
   Pass1 rel[2];
   double prefix_ms = 0, partition_ms = 0;
 
-  if (cpu_pass1) {
-    // pass 1 on the CPU, straight into pinned host memory
-    const uint32_t threads = cfg->cpu_threads ? cfg->cpu_threads : std::max(1u, std::thread::hardware_concurrency());
-    const auto t0 = std::chrono::steady_clock::now();
-    for (int r = 0; r < 2; ++r) {
-      Pass1& s = rel[r];
-      s.n = lens[r];
-      s.pad = pad;
-      s.padded_len = s.n + F1 * pad;
-      s.off.assign(F1, 0);
-      s.cut = 0;
-      s.cut_part = 0;
-      OOC_HIP(A.halloc(&s.spill, s.padded_len * tb));
-      cpu_partition_contiguous(kt, cols[r][0], cols[r][1], s.n, bits1, 0, threads, s.spill, s.off.data());
+  // ---- pass 2 + join over groups of consecutive pass-1 partitions -------
+  // A group holds at most `cap` tuples of either relation (a partition
+  // longer than that gets a group of its own); groups end at either
+  // relation's cache boundary, so a relation's part of a group is wholly
+  // cached (read in place) or wholly spilled (copied).
+  auto make_groups = [&](const Pass1& R, const Pass1& S, uint64_t cap, std::vector<Group>& groups) {
+    groups.clear();
+    for (uint32_t p = 0; p < F1;) {
+      Group g{};
+      g.p0 = p;
+      uint32_t q = p + 1;
+      while (q < F1 && q != R.cut_part && q != S.cut_part) {
+        if (R.end(q) - R.start(p) > cap || S.end(q) - S.start(p) > cap) break;
+        ++q;
+      }
+      g.p1 = q;
+      groups.push_back(g);
+      p = q;
     }
-    partition_ms = ms_since(t0);
+  };
+  // every group's partition starts relative to the group, both relations
+  // (2 F1 words in all)
+  auto group_offsets = [&](const Pass1& R, const Pass1& S, std::vector<Group>& groups, uint64_t* h_goff) {
+    uint64_t at = 0;
+    const Pass1* rs[2] = {&R, &S};
+    for (Group& g : groups) {
+      g.off_at = at;
+      for (int r = 0; r < 2; ++r) {
+        g.shift[r] = rs[r]->start(g.p0);
+        g.len[r] = rs[r]->end(g.p1 - 1) - g.shift[r];
+        for (uint32_t p = g.p0; p < g.p1; ++p) h_goff[at++] = rs[r]->off[p] - g.shift[r];
+      }
+    }
+  };
+  auto longest = [&](const Pass1& r) {
+    uint64_t m = 0;
+    for (uint32_t p = 0; p < F1; ++p) m = std::max(m, r.end(p) - r.start(p));
+    return m;
+  };
+
+  // two streams, each with the buffers of one group
+  constexpr int NS = 2;
+  const uint32_t tile2 = pipeline_tile2(kt);
+  PipelineDevice sd[NS];
+  hipStream_t st[NS];
+  for (int s = 0; s < NS; ++s) {
+    OOC_HIP(hipStreamCreateWithFlags(&st[s], hipStreamNonBlocking));
+    A.streams.push_back(st[s]);
+  }
+  std::vector<void*> stream_allocs;
+  struct StreamBuffers {  // freed when the join returns, before the streams
+    std::vector<void*>& allocs;
+    hipStream_t* st;
+    ~StreamBuffers() {
+      for (int s = 0; s < NS; ++s) (void)hipStreamSynchronize(st[s]);
+      for (void* p : allocs) (void)hipFree(p);
+    }
+  } stream_buffers{stream_allocs, st};
+  uint64_t cap = 0;
+  // (re)allocates both streams' buffers for groups of up to `want` tuples
+  auto size_streams = [&](uint64_t want) -> fij_status {
+    if (want <= cap) return FIJ_OK;
+    if (want + F1 * pad >= 0xFFFFFFFFull) return ooc_fail(FIJ_INTEGER_OVERFLOW, "a partition group exceeds 2^32 tuples");
+    for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamSynchronize(st[s]));
+    for (void* q : stream_allocs) OOC_HIP(hipFree(q));
+    stream_allocs.clear();
+    cap = want;
+    const uint32_t max_tiles = static_cast<uint32_t>(cap / tile2 + F1 + 1);
+    auto dal = [&](auto** ptr, uint64_t count) -> hipError_t {
+      void* v = nullptr;
+      const hipError_t e = hipMalloc(&v, std::max<uint64_t>(count, 1) * sizeof(**ptr));
+      if (e == hipSuccess) {
+        stream_allocs.push_back(v);
+        *ptr = static_cast<std::remove_reference_t<decltype(*ptr)>>(v);
+      }
+      return e;
+    };
+    for (int s = 0; s < NS; ++s) {
+      PipelineDevice& d = sd[s];
+      d = PipelineDevice{};
+      d.F2 = F2;
+      d.bits1 = bits1;
+      d.bits2 = bits2;
+      d.pad = static_cast<uint32_t>(pad);
+      d.tile2 = tile2;
+      d.sides = 3;
+      d.agg_len = join_grid;
+      for (PipelineSide* ps : {&d.build, &d.probe}) {
+        std::vector<void*> allocs;
+        const hipError_t e = alloc_pipeline_side(*ps, kt, cap, F1, F2, 1, true, max_tiles, allocs);
+        stream_allocs.insert(stream_allocs.end(), allocs.begin(), allocs.end());
+        OOC_HIP(e);
+      }
+      OOC_HIP(dal(&d.work_counter, kJoinCounterWords));
+      const uint64_t items = static_cast<uint64_t>(F1) * F2;
+      OOC_HIP(dal(&d.overflow, items));
+      // the deferred-item flags start at zero; every join leaves them at zero
+      OOC_HIP(hipMemsetAsync(d.overflow, 0, items, st[s]));
+      d.unit_cap = static_cast<uint32_t>(items + 2 * cap / 65536 + 2);
+      OOC_HIP(dal(&d.units, d.unit_cap));
+      OOC_HIP(dal(&d.aggregates, join_grid));
+    }
+    for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamSynchronize(st[s]));
+    return FIJ_OK;
+  };
+  // Enqueues pass 2 and the join of every group, alternating the streams;
+  // group gi's result goes to results[gi].
+  int rr = 0;
+  auto enqueue_groups = [&](const Pass1& R, const Pass1& S, const std::vector<Group>& groups, const uint64_t* h_goff,
+                            JoinAggregate* results) -> fij_status {
+    const Pass1* rs[2] = {&R, &S};
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const Group& g = groups[gi];
+      const int s = rr++ % NS;
+      PipelineDevice d = sd[s];
+      d.F1 = g.p1 - g.p0;
+      uint64_t at_r = g.off_at;
+      PipelineSide* sides[2] = {&d.build, &d.probe};
+      for (int r = 0; r < 2; ++r) {
+        PipelineSide& ps = *sides[r];
+        const Pass1& src = *rs[r];
+        ps.padded_len1 = g.len[r];
+        ps.n = g.len[r];
+        if (g.shift[r] + g.len[r] <= src.cut) {
+          ps.rel1 = src.cache + g.shift[r] * tb;  // cached: read in place
+        } else {
+          // spilled: one copy into this stream's staging buffer
+          uint8_t* dst = static_cast<uint8_t*>(r == 0 ? sd[s].build.rel1 : sd[s].probe.rel1);
+          OOC_HIP(hipMemcpyAsync(dst, src.spill + (g.shift[r] - src.cut) * tb, g.len[r] * tb, hipMemcpyHostToDevice,
+                                 st[s]));
+          ps.rel1 = dst;
+        }
+        OOC_HIP(hipMemcpyAsync(ps.off1, h_goff + at_r, 8ull * d.F1, hipMemcpyHostToDevice, st[s]));
+        at_r += d.F1;
+      }
+      OOC_HIP(launch_pipeline_plan(d, st[s]));
+      const uint32_t tiles = static_cast<uint32_t>((g.len[0] + g.len[1]) / tile2 + 2 * d.F1);
+      OOC_HIP(launch_pipeline_sort2(d, kt, std::max(1u, std::min(cus, tiles)), st[s]));
+      OOC_HIP(launch_pipeline_join(d, kt, false, false, join_grid, st[s]));
+      OOC_HIP(launch_pipeline_join(d, kt, false, true, join_grid, st[s]));
+      OOC_HIP(launch_sum_aggregates(d.aggregates, join_grid, results + gi, st[s]));
+    }
+    return FIJ_OK;
+  };
+  auto add_results = [&](const JoinAggregate* d_res, size_t n) -> fij_status {
+    std::vector<JoinAggregate> h(n);
+    if (n) OOC_HIP(hipMemcpy(h.data(), d_res, sizeof(JoinAggregate) * n, hipMemcpyDeviceToHost));
+    for (const JoinAggregate& a : h) {
+      result->count += a.count;
+      result->sum_build += a.sum_build;
+      result->sum_probe += a.sum_probe;
+    }
+    return FIJ_OK;
+  };
+  const uint64_t staging = std::max<uint64_t>(cfg->staging_tuples ? cfg->staging_tuples : (1ull << 24), 1ull << 16);
+  double join_ms = 0;
+  uint64_t n_groups = 0;
+
+  if (cpu_pass1) {
+    // Pass 1 on host threads, overlapped with the GPU as in the reference's
+    // CPU-partitioned join (cpu_partitioned_radix_join.rs:457-547: the build
+    // side's transfer and pass 2 run on the GPU while the CPU partitions the
+    // probe side):
+    //   1. the build relation R is partitioned into pinned host memory;
+    //   2. its first partitions, up to cache_bytes, are copied to device
+    //      memory by one DMA copy while
+    //   3. the probe relation S is partitioned in slices of its input, each
+    //      into its own contiguous pass-1 layout, and every slice is joined
+    //      with R on the GPU while the host threads partition the next one
+    //      (Join(R, S) is the sum of the Join(R, S_k)).
+    const uint32_t threads = cfg->cpu_threads ? cfg->cpu_threads : std::max(1u, std::thread::hardware_concurrency());
+    double cpu_ms = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    Pass1& R = rel[0];
+    R.n = build_len;
+    R.pad = pad;
+    R.padded_len = R.n + F1 * pad;
+    R.off.assign(F1, 0);
+    uint8_t* r_host = nullptr;
+    OOC_HIP(A.halloc(&r_host, R.padded_len * tb));
+    cpu_partition_contiguous(kt, build_keys, build_pays, R.n, bits1, 0, threads, r_host, R.off.data());
+    cpu_ms += ms_since(t0);
+    uint32_t P = 0;
+    while (P < F1 && R.end(P) * tb <= cfg->cache_bytes) ++P;
+    R.cut_part = P;
+    R.cut = P < F1 ? R.start(P) : R.padded_len;
+    R.spill = r_host + R.cut * tb;  // stands for tuple index R.cut
+    if (R.cut) {
+      OOC_HIP(A.dalloc(&R.cache, R.cut * tb));
+      OOC_HIP(hipMemcpyAsync(R.cache, r_host, R.cut * tb, hipMemcpyHostToDevice, main_st));
+      OOC_HIP(hipEventRecord(ev[3], main_st));
+      for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamWaitEvent(st[s], ev[3], 0));
+    }
+    // probe slices of about `staging` tuples, at least 1, at most 8
+    uint64_t K = cfg->probe_slices ? cfg->probe_slices : std::min<uint64_t>(8, std::max<uint64_t>(1, probe_len / staging));
+    K = std::max<uint64_t>(1, std::min<uint64_t>(K, probe_len));
+    const uint64_t slice_len = (probe_len + K - 1) / K;
+    // two slice buffers: slice k + 1 is partitioned while slice k's copies
+    // and kernels run
+    Pass1 slice[2];
+    uint64_t* h_goff[2] = {nullptr, nullptr};
+    hipEvent_t slice_free[2];
+    for (int b = 0; b < 2; ++b) {
+      OOC_HIP(A.halloc(&slice[b].spill, (slice_len + F1 * pad) * tb));
+      OOC_HIP(A.halloc(&h_goff[b], 2ull * F1));
+      OOC_HIP(hipEventCreate(&slice_free[b]));
+      A.events.push_back(slice_free[b]);
+    }
+    JoinAggregate* d_res = nullptr;
+    OOC_HIP(A.dalloc(&d_res, K * F1));
+    std::vector<Group> groups;
+    const size_t kb = key_bytes(kt);
+    for (uint64_t k = 0; k < K; ++k) {
+      const uint64_t b0 = std::min<uint64_t>(probe_len, k * slice_len);
+      const uint64_t n = std::min<uint64_t>(probe_len, b0 + slice_len) - b0;
+      Pass1& S = slice[k & 1];
+      if (k >= 2) OOC_HIP(hipEventSynchronize(slice_free[k & 1]));  // slice k - 2 has left this buffer
+      t0 = std::chrono::steady_clock::now();
+      S.n = n;
+      S.pad = pad;
+      S.padded_len = n + F1 * pad;
+      S.off.assign(F1, 0);
+      S.cut = 0;
+      S.cut_part = 0;
+      cpu_partition_contiguous(kt, static_cast<const uint8_t*>(probe_keys) + b0 * kb,
+                               static_cast<const uint8_t*>(probe_pays) + b0 * kb, n, bits1, 0, threads, S.spill,
+                               S.off.data());
+      cpu_ms += ms_since(t0);
+      fij_status fs = size_streams(std::max({staging, longest(R), longest(S)}));
+      if (fs != FIJ_OK) return fs;
+      make_groups(R, S, cap, groups);
+      group_offsets(R, S, groups, h_goff[k & 1]);
+      if (k + 1 == K) t0 = std::chrono::steady_clock::now();
+      fs = enqueue_groups(R, S, groups, h_goff[k & 1], d_res + n_groups);
+      if (fs != FIJ_OK) return fs;
+      n_groups += groups.size();
+      // the buffer is free again once both streams have passed this point
+      for (int s = 0; s < NS; ++s) OOC_HIP(hipEventRecord(ev[s], st[s]));
+      for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamWaitEvent(main_st, ev[s], 0));
+      OOC_HIP(hipEventRecord(slice_free[k & 1], main_st));
+    }
+    for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamSynchronize(st[s]));
+    join_ms = ms_since(t0);  // the last slice's transfers and kernels, after the host threads finished
+    partition_ms = cpu_ms;
+    fij_status fs = add_results(d_res, n_groups);
+    if (fs != FIJ_OK) return fs;
   } else {
     // pass 1 on the GPU: histogram and offsets, then the cache split, then
     // the scatter into cache and spill
@@ -291,120 +522,23 @@ extern "C" fij_status fij_ooc_join(const fij_ooc_config* cfg, const void* build_
     OOC_HIP(hipStreamSynchronize(main_st));
     OOC_HIP(hipEventElapsedTime(&f, ev[1], ev[2]));
     partition_ms = f;
-  }
-
-  // groups of consecutive pass-1 partitions: within the cache or within
-  // the spill, at most `cap` tuples of either relation (a partition longer
-  // than that gets a group of its own, and the capacity grows to it)
-  uint64_t cap = std::max<uint64_t>(cfg->staging_tuples ? cfg->staging_tuples : (1ull << 24), 1ull << 16);
-  for (int r = 0; r < 2; ++r)
-    for (uint32_t p = 0; p < F1; ++p) cap = std::max(cap, rel[r].end(p) - rel[r].start(p));
-  if (cap + F1 * pad >= 0xFFFFFFFFull) return ooc_fail(FIJ_INTEGER_OVERFLOW, "a partition group exceeds 2^32 tuples");
-  // Groups end at either relation's cache boundary, so a relation's part
-  // of a group is wholly cached (read in place) or wholly spilled (copied).
-  std::vector<Group> groups;
-  for (uint32_t p = 0; p < F1;) {
-    Group g{};
-    g.p0 = p;
-    uint32_t q = p + 1;
-    while (q < F1 && q != rel[0].cut_part && q != rel[1].cut_part) {
-      bool ok = true;
-      for (int r = 0; r < 2; ++r) ok = ok && rel[r].end(q) - rel[r].start(p) <= cap;
-      if (!ok) break;
-      ++q;
-    }
-    g.p1 = q;
-    groups.push_back(g);
-    p = q;
-  }
-  uint64_t* h_goff = nullptr;
-  OOC_HIP(A.halloc(&h_goff, 2ull * (F1 + groups.size())));
-  uint64_t at = 0;
-  for (Group& g : groups) {
-    g.off_at = at;
-    for (int r = 0; r < 2; ++r) {
-      g.shift[r] = rel[r].start(g.p0);
-      g.len[r] = rel[r].end(g.p1 - 1) - g.shift[r];
-      for (uint32_t p = g.p0; p < g.p1; ++p) h_goff[at++] = rel[r].off[p] - g.shift[r];
-    }
-  }
-
-  // two streams, each with the buffers of one group
-  constexpr int NS = 2;
-  const uint32_t tile2 = pipeline_tile2(kt);
-  const uint32_t max_tiles = static_cast<uint32_t>(cap / tile2 + F1 + 1);
-  JoinAggregate* d_results = nullptr;
-  OOC_HIP(A.dalloc(&d_results, groups.size()));
-  PipelineDevice sd[NS];
-  hipStream_t st[NS];
-  for (int s = 0; s < NS; ++s) {
-    OOC_HIP(hipStreamCreateWithFlags(&st[s], hipStreamNonBlocking));
-    A.streams.push_back(st[s]);
-    PipelineDevice& d = sd[s];
-    d = PipelineDevice{};
-    d.F2 = F2;
-    d.bits1 = bits1;
-    d.bits2 = bits2;
-    d.pad = static_cast<uint32_t>(pad);
-    d.tile2 = tile2;
-    d.sides = 3;
-    d.agg_len = join_grid;
-    for (PipelineSide* ps : {&d.build, &d.probe}) {
-      std::vector<void*> allocs;
-      const hipError_t e = alloc_pipeline_side(*ps, kt, cap, F1, F2, 1, true, max_tiles, allocs);
-      A.dev.insert(A.dev.end(), allocs.begin(), allocs.end());
-      OOC_HIP(e);
-    }
-    OOC_HIP(A.dalloc(&d.work_counter, kJoinCounterWords));
-    const uint64_t items = static_cast<uint64_t>(F1) * F2;
-    OOC_HIP(A.dalloc(&d.overflow, items));
-    OOC_HIP(hipMemsetAsync(d.overflow, 0, items, st[s]));
-    d.unit_cap = static_cast<uint32_t>(items + 2 * cap / 65536 + 2);
-    OOC_HIP(A.dalloc(&d.units, d.unit_cap));
-    OOC_HIP(A.dalloc(&d.aggregates, join_grid));
-  }
-  for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamSynchronize(st[s]));
-
-  const auto t_join = std::chrono::steady_clock::now();
-  for (size_t gi = 0; gi < groups.size(); ++gi) {
-    const Group& g = groups[gi];
-    const int s = static_cast<int>(gi % NS);
-    PipelineDevice d = sd[s];
-    d.F1 = g.p1 - g.p0;
-    uint64_t at_r = g.off_at;
-    PipelineSide* sides[2] = {&d.build, &d.probe};
-    for (int r = 0; r < 2; ++r) {
-      PipelineSide& ps = *sides[r];
-      const Pass1& src = rel[r];
-      ps.padded_len1 = g.len[r];
-      ps.n = g.len[r];
-      if (g.shift[r] + g.len[r] <= src.cut) {
-        ps.rel1 = src.cache + g.shift[r] * tb;  // cached: read in place
-      } else {
-        // spilled: one copy into this stream's staging buffer
-        uint8_t* dst = static_cast<uint8_t*>(r == 0 ? sd[s].build.rel1 : sd[s].probe.rel1);
-        OOC_HIP(hipMemcpyAsync(dst, src.spill + (g.shift[r] - src.cut) * tb, g.len[r] * tb, hipMemcpyHostToDevice,
-                               st[s]));
-        ps.rel1 = dst;
-      }
-      OOC_HIP(hipMemcpyAsync(ps.off1, h_goff + at_r, 8ull * d.F1, hipMemcpyHostToDevice, st[s]));
-      at_r += d.F1;
-    }
-    OOC_HIP(launch_pipeline_plan(d, st[s]));
-    const uint32_t tiles = static_cast<uint32_t>((g.len[0] + g.len[1]) / tile2 + 2 * d.F1);
-    OOC_HIP(launch_pipeline_sort2(d, kt, std::max(1u, std::min(cus, tiles)), st[s]));
-    OOC_HIP(launch_pipeline_join(d, kt, false, false, join_grid, st[s]));
-    OOC_HIP(launch_pipeline_join(d, kt, false, true, join_grid, st[s]));
-    OOC_HIP(launch_sum_aggregates(d.aggregates, join_grid, d_results + gi, st[s]));
-  }
-  for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamSynchronize(st[s]));
-  const double join_ms = ms_since(t_join);
-  std::vector<JoinAggregate> h(groups.size());
-  OOC_HIP(hipMemcpy(h.data(), d_results, sizeof(JoinAggregate) * groups.size(), hipMemcpyDeviceToHost));
-  for (const JoinAggregate& a : h) {
-    result->count += a.count;
-    result->sum_build += a.sum_build;
-    result->sum_probe += a.sum_probe;
+    fij_status fs = size_streams(std::max({staging, longest(rel[0]), longest(rel[1])}));
+    if (fs != FIJ_OK) return fs;
+    std::vector<Group> groups;
+    make_groups(rel[0], rel[1], cap, groups);
+    uint64_t* h_goff = nullptr;
+    OOC_HIP(A.halloc(&h_goff, 2ull * F1));
+    group_offsets(rel[0], rel[1], groups, h_goff);
+    JoinAggregate* d_res = nullptr;
+    OOC_HIP(A.dalloc(&d_res, groups.size()));
+    const auto t_join = std::chrono::steady_clock::now();
+    fs = enqueue_groups(rel[0], rel[1], groups, h_goff, d_res);
+    if (fs != FIJ_OK) return fs;
+    for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamSynchronize(st[s]));
+    join_ms = ms_since(t_join);
+    fs = add_results(d_res, groups.size());
+    if (fs != FIJ_OK) return fs;
+    n_groups = groups.size();
   }
   if (timing) {
     timing->prefix_sum_ms = static_cast<float>(prefix_ms);
@@ -413,7 +547,7 @@ extern "C" fij_status fij_ooc_join(const fij_ooc_config* cfg, const void* build_
     timing->total_ms = static_cast<float>(ms_since(t_start));
     timing->cached_build_tuples = rel[0].cut;
     timing->cached_probe_tuples = rel[1].cut;
-    timing->groups = static_cast<uint32_t>(groups.size());
+    timing->groups = static_cast<uint32_t>(n_groups);
   }
   return FIJ_OK;
 }

@@ -83,6 +83,102 @@ __global__ __launch_bounds__(1024) void scatter_direct_kernel(ScatterParams p) {
   }
 }
 
+// High-fanout scatter (fanouts whose line rings do not fit LDS): the chunk
+// is read in tiles of ITEMS * BLOCK tuples; a tile is counting-sorted by
+// digit in LDS (ranks from LDS atomics on the tile's counters, a block scan)
+// and written out in sorted order, so that the lanes of a store write runs
+// of consecutive addresses -- one run per digit of the tile -- instead of
+// one address per lane.  Partial lines are merged in the L2 and the memory
+// side cache before they reach HBM.  The next tile is loaded while the
+// current one is sorted and written.
+template <typename K, uint32_t ITEMS, uint32_t BLOCK, bool SPILL>
+__global__ __launch_bounds__(BLOCK) void scatter_tile_kernel(ScatterParams p) {
+  using T = Tuple<K>;
+  constexpr uint32_t TILE = ITEMS * BLOCK;
+  extern __shared__ __attribute__((aligned(16))) uint64_t lds_u64[];
+  __shared__ uint32_t wave_tmp[BLOCK / kWave + 1];
+  const uint32_t F = 1u << p.radix_bits;
+  const uint64_t mask = radix_mask(p.radix_bits, p.ignore_bits);
+  const uint32_t tid = threadIdx.x;
+  T* sorted = reinterpret_cast<T*>(lds_u64);               // [TILE]
+  uint32_t* start = reinterpret_cast<uint32_t*>(sorted + TILE);  // [F] tile counts, then run starts
+  uint32_t* cursor = start + F;                             // [F] next position, relative to cbase
+  const uint64_t* off = p.chunk_offsets + static_cast<uint64_t>(blockIdx.x) * F;
+  const uint64_t cbase = off[0];
+  for (uint32_t i = tid; i < F; i += BLOCK) {
+    cursor[i] = static_cast<uint32_t>(off[i] - cbase);
+    start[i] = 0;
+  }
+  uint64_t begin, len;
+  chunk_range<K>(p, blockIdx.x, gridDim.x, begin, len);
+  const K* keys = static_cast<const K*>(p.keys) + begin;
+  const K* pays = static_cast<const K*>(p.pays) + begin;
+  T* rel = static_cast<T*>(p.relation) + cbase;
+  __syncthreads();
+  if (len == 0) return;  // uniform
+  // lanes past the chunk's end re-read its last tuple and are ignored
+  auto load = [&](uint64_t t0, T (&v)[ITEMS]) {
+#pragma unroll
+    for (uint32_t k = 0; k < ITEMS; ++k) {
+      const uint64_t e = min(t0 + k * BLOCK + tid, len - 1);
+      v[k].key = __builtin_nontemporal_load(keys + e);
+      v[k].value = __builtin_nontemporal_load(pays + e);
+    }
+  };
+  auto process = [&](uint64_t t0, const T (&v)[ITEMS]) {
+    const uint32_t n = static_cast<uint32_t>(min<uint64_t>(TILE, len - t0));
+    uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+    for (uint32_t k = 0; k < ITEMS; ++k) {
+      const bool act = k * BLOCK + tid < n;
+      dig[k] = key_to_partition<K>(v[k].key, mask, p.ignore_bits);
+      rank[k] = wave_rank_add(start, dig[k], act);
+    }
+    __syncthreads();
+    block_exclusive_scan<uint32_t>(start, F, wave_tmp);
+#pragma unroll
+    for (uint32_t k = 0; k < ITEMS; ++k)
+      if (k * BLOCK + tid < n) sorted[start[dig[k]] + rank[k]] = v[k];
+    __syncthreads();
+    // element i of the sorted tile is element i - start[d] of digit d's run
+    for (uint32_t i = tid; i < n; i += BLOCK) {
+      const T x = sorted[i];
+      const uint32_t d = key_to_partition<K>(x.key, mask, p.ignore_bits);
+      const uint32_t pos = cursor[d] + (i - start[d]);
+      store_tuple<K>(as_global(part_dst<SPILL>(p, rel, cbase, d) + pos), x.key, x.value);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < F; i += BLOCK) {
+      const uint32_t end = i + 1 < F ? start[i + 1] : n;
+      cursor[i] += end - start[i];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < F; i += BLOCK) start[i] = 0;
+    // (the next tile's ranks are taken after the barrier inside process)
+  };
+  T va[ITEMS], vb[ITEMS];
+  uint64_t t0 = 0;
+  load(t0, va);
+  auto step = [&](T (&cbuf)[ITEMS], T (&nbuf)[ITEMS]) -> bool {
+    const uint64_t tn = t0 + TILE;
+    const bool more = tn < len;
+    load(more ? tn : t0, nbuf);
+    process(t0, cbuf);
+    __syncthreads();
+    t0 = tn;
+    return more;
+  };
+  for (;;) {
+    if (!step(va, vb)) break;
+    if (!step(vb, va)) break;
+  }
+}
+
+template <typename K, uint32_t ITEMS, uint32_t BLOCK>
+constexpr uint32_t tile_scatter_lds(uint32_t fanout) {
+  return ITEMS * BLOCK * sizeof(Tuple<K>) + 2 * 4 * fanout;
+}
+
 // Steps whose loads a loader has in flight.
 constexpr uint32_t kScatterPrefetch = 8;
 
@@ -336,10 +432,10 @@ __global__ __launch_bounds__(BLOCK) void scatter_swwc_kernel(ScatterParams p) {
 
 // Many relations in one launch: block b scatters chunk b - prefix[j] of
 // relation j (batched joins; every relation has the same fanout).
-template <typename K>
-__global__ __launch_bounds__(1024) void scatter_swwc_batched_kernel(const ScatterParams* __restrict__ jobs,
-                                                                    const uint32_t* __restrict__ prefix,
-                                                                    uint32_t njobs) {
+template <typename K, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void scatter_swwc_batched_kernel(const ScatterParams* __restrict__ jobs,
+                                                                     const uint32_t* __restrict__ prefix,
+                                                                     uint32_t njobs) {
   uint32_t lo = 0, hi = njobs;  // the last j with prefix[j] <= blockIdx.x
   while (hi - lo > 1) {
     const uint32_t m = (lo + hi) / 2;
@@ -349,7 +445,7 @@ __global__ __launch_bounds__(1024) void scatter_swwc_batched_kernel(const Scatte
   // loaded from them as global)
   const ScatterParams p = jobs[lo];
   const uint32_t c0 = scalar_load(prefix + lo);
-  scatter_swwc_body<K, 1024, false>(p, blockIdx.x - c0, scalar_load(prefix + lo + 1) - c0);
+  scatter_swwc_body<K, BLOCK, false>(p, blockIdx.x - c0, scalar_load(prefix + lo + 1) - c0);
 }
 
 // Queue entries per step buffer: a step completes at most one line per
@@ -394,6 +490,33 @@ hipError_t launch_swwc_t(ScatterParams p, uint32_t fanout, uint32_t chunks, uint
   return hipGetLastError();
 }
 
+// Launches the tile-sorting scatter with the largest tile (4096 or 2048
+// tuples of 16 bytes, 8192 or 4096 of 8 bytes) whose LDS fits; returns
+// hipErrorNotSupported without launching when none does.
+template <typename K, uint32_t ITEMS, bool SPILL>
+hipError_t launch_tile_items(const ScatterParams& p, uint32_t fanout, uint32_t chunks, uint32_t lds_total,
+                             hipStream_t stream) {
+  auto kern = scatter_tile_kernel<K, ITEMS, 1024, SPILL>;
+  const uint32_t budget = dynamic_lds_budget(reinterpret_cast<const void*>(kern), lds_total);
+  const uint32_t lds = tile_scatter_lds<K, ITEMS, 1024>(fanout);
+  if (lds > budget) return hipErrorNotSupported;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds));
+  hipLaunchKernelGGL(kern, dim3(chunks), dim3(1024), lds, stream, p);
+  return hipGetLastError();
+}
+
+template <typename K>
+hipError_t launch_tile_t(const ScatterParams& p, uint32_t fanout, uint32_t chunks, uint32_t lds_total, bool spill,
+                         hipStream_t stream) {
+  constexpr uint32_t I = sizeof(K) == 8 ? 4 : 8;
+  hipError_t e = spill ? launch_tile_items<K, I, true>(p, fanout, chunks, lds_total, stream)
+                       : launch_tile_items<K, I, false>(p, fanout, chunks, lds_total, stream);
+  if (e != hipErrorNotSupported) return e;
+  return spill ? launch_tile_items<K, I / 2, true>(p, fanout, chunks, lds_total, stream)
+               : launch_tile_items<K, I / 2, false>(p, fanout, chunks, lds_total, stream);
+}
+
 template <typename K>
 hipError_t launch_partition_t(const PartitionLaunch& a, hipStream_t stream) {
   ScatterParams p{};
@@ -419,6 +542,9 @@ hipError_t launch_partition_t(const PartitionLaunch& a, hipStream_t stream) {
       e = launch_swwc_t<K, 512, false>(p, F, a.chunks, a.lds_bytes, stream);
     else
       e = launch_swwc_t<K, 256, false>(p, F, a.chunks, a.lds_bytes, stream);
+    if (e != hipErrorNotSupported) return e;
+    // no line ring fits: the tile-sorting scatter
+    e = launch_tile_t<K>(p, F, a.chunks, a.lds_bytes, spill, stream);
     if (e != hipErrorNotSupported) return e;
   }
   // Direct scatter: per-partition cursors only (4 bytes per partition).
@@ -446,58 +572,86 @@ hipError_t launch_partition(const PartitionLaunch& a, hipStream_t stream) {
 namespace {
 
 template <typename K>
-const void* batched_scatter_kernel() {
-  return reinterpret_cast<const void*>(scatter_swwc_batched_kernel<K>);
+const void* batched_scatter_kernel(uint32_t block) {
+  return block == 512 ? reinterpret_cast<const void*>(scatter_swwc_batched_kernel<K, 512>)
+                      : reinterpret_cast<const void*>(scatter_swwc_batched_kernel<K, 1024>);
 }
 
-// Deepest ring of the batched scatter for this fanout (0: none fits).
+// Ring of the batched scatter for this fanout and block: the deepest that
+// fits LDS, at most ring_cap lines when ring_cap is set (0: none fits).
 template <typename K>
-uint32_t batched_ring(uint32_t fanout, uint32_t lds_total) {
-  const uint32_t budget = dynamic_lds_budget(batched_scatter_kernel<K>(), lds_total);
-  for (uint32_t r = 256; r >= 1; r /= 2)
-    if (swwc_bytes(fanout, r, swwc_queue_len<K>(fanout, r, 1024)) <= budget) return r;
+uint32_t batched_ring(uint32_t fanout, uint32_t lds_total, uint32_t block, uint32_t ring_cap) {
+  const uint32_t budget = dynamic_lds_budget(batched_scatter_kernel<K>(block), lds_total);
+  const uint32_t top = ring_cap ? (ring_cap < 256 ? ring_cap : 256) : 256;
+  for (uint32_t r = top; r >= 1; r /= 2)
+    if (swwc_bytes(fanout, r, swwc_queue_len<K>(fanout, r, block)) <= budget) return r;
   return 0;
+}
+
+template <typename K>
+hipError_t scatter_params_batched_t(const void* keys, const void* pays, uint64_t n, uint32_t chunks,
+                                    uint32_t radix_bits, const uint64_t* chunk_offsets, void* relation,
+                                    uint32_t lds_total, uint32_t block, uint32_t ring_cap, ScatterParams* out) {
+  const uint32_t F = 1u << radix_bits;
+  const uint32_t ring = batched_ring<K>(F, lds_total, block, ring_cap);
+  if (ring == 0) return hipErrorNotSupported;
+  ScatterParams p{};
+  p.keys = keys;
+  p.pays = pays;
+  p.n = n;
+  p.chunk_len = canonical_chunk_len<K>(n, chunks);
+  p.radix_bits = radix_bits;
+  p.ignore_bits = 0;
+  p.chunk_offsets = chunk_offsets;
+  p.relation = relation;
+  p.ring_lines = ring;
+  p.queue_len = swwc_queue_len<K>(F, ring, block);
+  p.spill_from = F;
+  *out = p;
+  return hipSuccess;
+}
+
+template <typename K>
+hipError_t launch_partition_batched_t(const ScatterParams* jobs, const uint32_t* prefix, uint32_t njobs,
+                                      uint32_t blocks, uint32_t radix_bits, uint32_t lds_total, uint32_t block,
+                                      uint32_t ring_cap, hipStream_t stream) {
+  const uint32_t F = 1u << radix_bits;
+  const uint32_t ring = batched_ring<K>(F, lds_total, block, ring_cap);
+  if (ring == 0) return hipErrorNotSupported;
+  const uint32_t lds = swwc_bytes(F, ring, swwc_queue_len<K>(F, ring, block));
+  (void)hipFuncSetAttribute(batched_scatter_kernel<K>(block), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds));
+  if (block == 512)
+    hipLaunchKernelGGL((scatter_swwc_batched_kernel<K, 512>), dim3(blocks), dim3(512), lds, stream, jobs, prefix, njobs);
+  else
+    hipLaunchKernelGGL((scatter_swwc_batched_kernel<K, 1024>), dim3(blocks), dim3(1024), lds, stream, jobs, prefix,
+                       njobs);
+  return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t scatter_params_batched(KeyType t, const void* keys, const void* pays, uint64_t n, uint32_t chunks,
                                   uint32_t radix_bits, const uint64_t* chunk_offsets, void* relation,
-                                  uint32_t lds_total, ScatterParams* out) {
-  const uint32_t F = 1u << radix_bits;
-  const uint32_t ring = t == KeyType::I32 ? batched_ring<int32_t>(F, lds_total) : batched_ring<int64_t>(F, lds_total);
-  if (ring == 0) return hipErrorNotSupported;
-  ScatterParams p{};
-  p.keys = keys;
-  p.pays = pays;
-  p.n = n;
-  p.chunk_len = t == KeyType::I32 ? canonical_chunk_len<int32_t>(n, chunks) : canonical_chunk_len<int64_t>(n, chunks);
-  p.radix_bits = radix_bits;
-  p.ignore_bits = 0;
-  p.chunk_offsets = chunk_offsets;
-  p.relation = relation;
-  p.ring_lines = ring;
-  p.queue_len = t == KeyType::I32 ? swwc_queue_len<int32_t>(F, ring, 1024) : swwc_queue_len<int64_t>(F, ring, 1024);
-  p.spill_from = F;
-  *out = p;
-  return hipSuccess;
+                                  uint32_t lds_total, uint32_t block, uint32_t ring_cap, ScatterParams* out) {
+  if (block != 512 && block != 1024) return hipErrorInvalidValue;
+  if (t == KeyType::I32)
+    return scatter_params_batched_t<int32_t>(keys, pays, n, chunks, radix_bits, chunk_offsets, relation, lds_total,
+                                             block, ring_cap, out);
+  return scatter_params_batched_t<int64_t>(keys, pays, n, chunks, radix_bits, chunk_offsets, relation, lds_total,
+                                           block, ring_cap, out);
 }
 
 hipError_t launch_partition_batched(KeyType t, const ScatterParams* jobs, const uint32_t* prefix, uint32_t njobs,
-                                    uint32_t blocks, uint32_t radix_bits, uint32_t lds_total, hipStream_t stream) {
+                                    uint32_t blocks, uint32_t radix_bits, uint32_t lds_total, uint32_t block,
+                                    uint32_t ring_cap, hipStream_t stream) {
   if (blocks == 0) return hipSuccess;
-  const uint32_t F = 1u << radix_bits;
-  const uint32_t ring = t == KeyType::I32 ? batched_ring<int32_t>(F, lds_total) : batched_ring<int64_t>(F, lds_total);
-  if (ring == 0) return hipErrorNotSupported;
-  const uint32_t ql = t == KeyType::I32 ? swwc_queue_len<int32_t>(F, ring, 1024) : swwc_queue_len<int64_t>(F, ring, 1024);
-  const uint32_t lds = swwc_bytes(F, ring, ql);
-  const void* kern = t == KeyType::I32 ? batched_scatter_kernel<int32_t>() : batched_scatter_kernel<int64_t>();
-  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (block != 512 && block != 1024) return hipErrorInvalidValue;
   if (t == KeyType::I32)
-    hipLaunchKernelGGL(scatter_swwc_batched_kernel<int32_t>, dim3(blocks), dim3(1024), lds, stream, jobs, prefix, njobs);
-  else
-    hipLaunchKernelGGL(scatter_swwc_batched_kernel<int64_t>, dim3(blocks), dim3(1024), lds, stream, jobs, prefix, njobs);
-  return hipGetLastError();
+    return launch_partition_batched_t<int32_t>(jobs, prefix, njobs, blocks, radix_bits, lds_total, block, ring_cap,
+                                               stream);
+  return launch_partition_batched_t<int64_t>(jobs, prefix, njobs, blocks, radix_bits, lds_total, block, ring_cap,
+                                             stream);
 }
 
 }  // namespace fij

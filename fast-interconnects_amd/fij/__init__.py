@@ -830,10 +830,43 @@ _proto("fij_join_batch_device_bytes", C.c_int, _vp, C.POINTER(_u64))
 
 
 def autotune(dtype, build_len, probe_len):
-    """(small, radix_bits_1, radix_bits_2) of the size-class rule."""
+    """(small, radix_bits_1, radix_bits_2) of the pair's size class (tuning
+    table entry, or the closed-form rule)."""
     sm, b1, b2 = _u32(), _u32(), _u32()
     check(lib.fij_join_autotune(int(_ktype(dtype)), build_len, probe_len, C.byref(sm), C.byref(b1), C.byref(b2)))
     return bool(sm.value), b1.value, b2.value
+
+
+class JoinTuning(C.Structure):
+    _fields_ = [("build_log2", _u32), ("probe_log2", _u32), ("small", _u32), ("radix_bits_1", _u32),
+                ("radix_bits_2", _u32), ("ring_lines", _u32), ("block", _u32), ("from_table", _u32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_proto("fij_join_tuning_lookup", C.c_int, C.c_int, _u64, _u64, C.POINTER(JoinTuning))
+_proto("fij_join_tuning_set", C.c_int, C.c_int, C.POINTER(JoinTuning), _u32, C.c_int)
+
+
+def tuning_lookup(dtype, build_len, probe_len):
+    """The configuration batched joins of these lengths run with (a dict of
+    fij_join_tuning's fields)."""
+    t = JoinTuning()
+    check(lib.fij_join_tuning_lookup(int(_ktype(dtype)), build_len, probe_len, C.byref(t)))
+    return t.as_dict()
+
+
+def tuning_set(dtype, entries=None, builtin=False):
+    """Replaces the active tuning table of `dtype` with `entries` (dicts of
+    fij_join_tuning fields; [] = the closed-form rule only), or restores the
+    built-in table."""
+    entries = entries or []
+    arr = (JoinTuning * max(len(entries), 1))()
+    for i, e in enumerate(entries):
+        for k, _ in JoinTuning._fields_:
+            setattr(arr[i], k, int(e.get(k, 0)))
+    check(lib.fij_join_tuning_set(int(_ktype(dtype)), arr, len(entries), 1 if builtin else 0))
 
 
 class JoinBatch:
@@ -1069,7 +1102,7 @@ def cpu_radix_partition(keys, payloads, radix_bits, ignore_bits=0, threads=0, ou
 
 class OocConfig(C.Structure):
     _fields_ = [("type", _u32), ("radix_bits_1", _u32), ("radix_bits_2", _u32), ("pass1_cpu", _u32),
-                ("cpu_threads", _u32), ("reserved", _u32), ("cache_bytes", _u64), ("staging_tuples", _u64)]
+                ("cpu_threads", _u32), ("probe_slices", _u32), ("cache_bytes", _u64), ("staging_tuples", _u64)]
 
 
 class OocTiming(C.Structure):
@@ -1093,11 +1126,11 @@ def _col_ptr(x, dtype):
 
 
 def ooc_join(dtype, bk, bp, sk, sp, bits1, bits2, pass1_cpu=False, cache_bytes=0, staging_tuples=0,
-             cpu_threads=0):
+             cpu_threads=0, probe_slices=0):
     """Out-of-core radix join; columns are numpy arrays (CPU pass 1 only),
     HostArray or DeviceArray.  Returns ((count, sum_build, sum_probe), timing)."""
-    cfg = OocConfig(int(_ktype(dtype)), bits1, bits2, 1 if pass1_cpu else 0, cpu_threads, 0, cache_bytes,
-                    staging_tuples)
+    cfg = OocConfig(int(_ktype(dtype)), bits1, bits2, 1 if pass1_cpu else 0, cpu_threads, probe_slices,
+                    cache_bytes, staging_tuples)
     want = numpy_dtype(_ktype(dtype))
     (bkp, nb), (bpp, nbp), (skp, ns), (spp, nsp) = (_col_ptr(x, want) for x in (bk, bp, sk, sp))
     if nbp != nb or nsp != ns:

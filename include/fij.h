@@ -390,10 +390,39 @@ typedef struct fij_join_batch_stats {
 
 typedef struct fij_join_batch fij_join_batch;
 
-/* The size-class rule: small = 1 when one block joins the pair from LDS;
- * otherwise the plan's radix bits for these lengths. */
+/* The size-class choice: small = 1 when one block joins the pair from LDS;
+ * otherwise the plan's radix bits for these lengths.  Looked up in the
+ * tuning table (fij_join_tuning_lookup); lengths of a class the table does
+ * not hold use the closed-form rule. */
 fij_status fij_join_autotune(fij_type type, uint64_t build_len, uint64_t probe_len, uint32_t *small,
                              uint32_t *radix_bits_1, uint32_t *radix_bits_2);
+
+/* Size-class tuning (no counterpart in the reference, whose radix bits and
+ * buffer sizes are fixed per run by scripts/benchmark_gpu_radix_join.py:55-67
+ * and sql-ops/build.rs:65-67).  A class is (ceil(log2 |R|), ceil(log2 |S|));
+ * its entry holds the configuration measured fastest for batched joins of
+ * that class on this GPU: the small-join path or the two-pass plan with its
+ * radix bits, the pass-1 line buffer depth and the pass-1 block shape.  The
+ * library carries a built-in table measured on MI355X
+ * (fast-interconnects_amd/autotune.py writes it). */
+typedef struct fij_join_tuning {
+  uint32_t build_log2;   /* class: ceil(log2 build_len) */
+  uint32_t probe_log2;   /* class: ceil(log2 probe_len) */
+  uint32_t small;        /* 1: one block joins the pair from an LDS table */
+  uint32_t radix_bits_1; /* two-pass plan: pass-1 and pass-2 radix bits */
+  uint32_t radix_bits_2;
+  uint32_t ring_lines;   /* pass-1 line buffer: 128-byte lines per partition, 0 = deepest that fits LDS */
+  uint32_t block;        /* pass-1 threads per block: 512 or 1024 */
+  uint32_t from_table;   /* lookup output: 1 = a table entry, 0 = the closed-form rule */
+} fij_join_tuning;
+
+/* The configuration batched joins of these lengths run with. */
+fij_status fij_join_tuning_lookup(fij_type type, uint64_t build_len, uint64_t probe_len, fij_join_tuning *out);
+/* Replaces the active table of `type` with entries[0..n) (n = 0: the rule
+ * only); a later call with use_builtin = 1 restores the built-in table.
+ * Entries are validated (bits, block, ring); batches executed afterwards use
+ * the new table. */
+fij_status fij_join_tuning_set(fij_type type, const fij_join_tuning *entries, uint32_t n, int use_builtin);
 fij_status fij_join_batch_create(const fij_join_batch_config *config, fij_join_batch **out);
 fij_status fij_join_batch_destroy(fij_join_batch *batch);
 /* Joins joins[0..n) (device columns) and writes results[0..n) (host).  A
@@ -473,8 +502,11 @@ fij_status fij_cpu_radix_partition(fij_type type, const void *keys, const void *
  * cpu_partitioned_radix_join.rs:118-769).  Pass 1 runs on the GPU, reading
  * the relations in place (device or pinned host memory), or on host threads
  * (any host memory); its partitions are cached in device memory up to
- * cache_bytes (split between the relations in proportion to their lengths;
- * GPU pass 1 only) and spilled to pinned host memory beyond it.  Pass 2 and
+ * cache_bytes (GPU pass 1: split between the relations in proportion to
+ * their lengths; CPU pass 1: the build relation's, copied while the host
+ * threads partition the probe relation slice by slice, every slice joined
+ * on the GPU while the next is partitioned) and spilled to pinned host
+ * memory beyond it.  Pass 2 and
  * the join then run over groups of consecutive pass-1 partitions (at most
  * staging_tuples tuples of either relation, or one partition when a single
  * one is longer) on two streams, spilled groups copied into device memory
@@ -486,7 +518,9 @@ typedef struct fij_ooc_config {
   uint32_t radix_bits_2;   /* 1..12 */
   uint32_t pass1_cpu;      /* 0: pass 1 on the GPU, 1: on host threads */
   uint32_t cpu_threads;    /* host threads of a CPU pass 1 (0 = all) */
-  uint32_t reserved;
+  uint32_t probe_slices;   /* CPU pass 1: the probe relation is partitioned in this many slices of its
+                              input, each joined on the GPU while the next is partitioned (0 = one per
+                              staging_tuples, at most 8) */
   uint64_t cache_bytes;    /* device memory for pass-1 partitions */
   uint64_t staging_tuples; /* group size (0 = 2^24) */
 } fij_ooc_config;

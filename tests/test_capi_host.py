@@ -285,8 +285,17 @@ def test_np_build_table_too_small():
 # ---- host-side rules and argument checks of the batch, server and
 # out-of-core entry points (no device work)
 
-def test_autotune_rule():
-    import math
+@pytest.fixture
+def rule_only():
+    """The closed-form size-class rule, without the built-in tuning table."""
+    fij.tuning_set(np.int32, [])
+    fij.tuning_set(np.int64, [])
+    yield
+    fij.tuning_set(np.int32, builtin=True)
+    fij.tuning_set(np.int64, builtin=True)
+
+
+def test_autotune_rule(rule_only):
     assert fij.autotune(np.int32, 1000, 16000)[0]          # one block's LDS table
     assert not fij.autotune(np.int32, 12289, 16000)[0]      # build side beyond one table fill
     assert not fij.autotune(np.int32, 1000, (1 << 22) + 1)[0]
@@ -302,6 +311,49 @@ def test_autotune_rule():
             assert (n_r * f) >> (b1 + b2) <= 512 or (b1, b2) == (9, 12)
             # a pass-1 partition's probe side spans at most 64 pass-2 tiles of 8192
             assert (n_r * f) >> b1 <= 64 * 8192 or b1 == 9
+
+
+def test_builtin_tuning_table():
+    """Every built-in entry is a valid configuration and is what batches of
+    its class run with; classes outside the table get the rule."""
+    import re
+
+    text = open(os.path.join(REPO, "fast-interconnects_amd", "csrc", "tuning_table.inc")).read()
+    for name, dtype in (("kTunedI32", np.int32), ("kTunedI64", np.int64)):
+        body = text[text.index(name):]
+        body = body[:body.index("};")]
+        rows = [tuple(int(x) for x in m.groups()) for m in
+                re.finditer(r"\{(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), 0\}", body)]
+        assert rows[-1] == (0,) * 7  # terminator
+        for bl, pl, small, b1, b2, ring, block in rows[:-1]:
+            assert 1 <= b1 <= 9 and 1 <= b2 <= 12 and block in (512, 1024) and ring in (0, 1, 2, 4, 8, 16, 32, 64, 128, 256)
+            n_r, n_s = (3 << bl) // 4, (3 << pl) // 4  # lengths inside the class
+            t = fij.tuning_lookup(dtype, n_r, n_s)
+            assert t["from_table"] == 1 and (t["build_log2"], t["probe_log2"]) == (bl, pl)
+            assert (t["radix_bits_1"], t["radix_bits_2"], t["ring_lines"], t["block"]) == (b1, b2, ring, block)
+            assert t["small"] == small
+    t = fij.tuning_lookup(np.int64, 1 << 35, 1 << 36)
+    assert t["from_table"] == 0 and (t["radix_bits_1"], t["radix_bits_2"]) == (9, 12)
+
+
+def test_tuning_set_validates_entries():
+    bad = [dict(build_log2=20, probe_log2=23, radix_bits_1=10, radix_bits_2=5, block=1024),
+           dict(build_log2=20, probe_log2=23, radix_bits_1=5, radix_bits_2=13, block=1024),
+           dict(build_log2=20, probe_log2=23, radix_bits_1=5, radix_bits_2=5, block=256),
+           dict(build_log2=20, probe_log2=23, radix_bits_1=5, radix_bits_2=5, block=512, ring_lines=3)]
+    try:
+        for e in bad:
+            with pytest.raises(fij.FijError) as ex:
+                fij.tuning_set(np.int32, [e])
+            assert ex.value.status == fij.Status.INVALID_ARGUMENT
+        fij.tuning_set(np.int32, [dict(build_log2=20, probe_log2=23, small=1, radix_bits_1=6, radix_bits_2=5,
+                                       ring_lines=8, block=512)])
+        t = fij.tuning_lookup(np.int32, 1 << 20, 1 << 23)
+        assert (t["radix_bits_1"], t["radix_bits_2"], t["ring_lines"], t["block"], t["from_table"]) == (6, 5, 8, 512, 1)
+        assert t["small"] == 0  # a 2^20 build side does not qualify for the small path
+        assert fij.autotune(np.int32, 1 << 20, 1 << 23) == (False, 6, 5)
+    finally:
+        fij.tuning_set(np.int32, builtin=True)
 
 
 def test_server_config_rejected_on_host():

@@ -145,3 +145,40 @@ def test_plan_enqueue_sized_matches_execute():
         assert got == want
     with pytest.raises(ValueError):  # shorter columns than the plan's: refused before any launch
         plan.execute(*d)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_every_tuning_candidate_gives_the_same_results(dtype):
+    """The autotuner's grid for a size class (radix bits moved by one either
+    way and split three ways, ring caps 0 and 4, blocks 512 and 1024, and the
+    small path where it qualifies): every configuration joins every pair of
+    the class exactly as the oracle does."""
+    sys_path_pkg()
+    import autotune
+
+    n_r, factor = 6000, 10  # class (13, 16): qualifies for the small path
+    host = [make_join(i, n_r, factor, dtype, seed=77) for i in range(6)]
+    dev = [tuple(to_dev(a) for a in j) for j in host]
+    want = [orc.sort_merge_join(*h) for h in host]
+    rule = fij.tuning_lookup(dtype, n_r, n_r * factor)
+    cands = autotune.candidates(rule, n_r, n_r * factor) if dtype == np.int32 else [
+        c for c in autotune.candidates(rule, n_r, n_r * factor) if not c["small"]]
+    assert len(cands) >= 20
+    try:
+        for cfg in cands:
+            fij.tuning_set(dtype, [dict(cfg, build_log2=rule["build_log2"], probe_log2=rule["probe_log2"])])
+            got, stats = fij.JoinBatch(dtype, len(dev)).execute(dev)
+            assert got == want, cfg
+            assert stats["small_joins"] == (len(dev) if cfg["small"] else 0), cfg
+    finally:
+        fij.tuning_set(dtype, builtin=True)
+    assert fij.device_status() == 0
+
+
+def sys_path_pkg():
+    import os
+    import sys
+
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fast-interconnects_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)

@@ -49,6 +49,21 @@ def test_gpu_pass1_from_pinned_memory(dtype, cache_frac, staging):
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_gpu_pass1_high_fanout_spill(dtype):
+    """Pass 1 at 11 bits (the tile-sorting scatter) writing part of the
+    partitions to the device cache and the rest to pinned host memory."""
+    h = make(1 << 18, 1 << 20, dtype, 6)
+    want = orc.sort_merge_join(*h)
+    cols = [fij.HostArray.from_numpy(a) for a in h]
+    tb = 2 * np.dtype(dtype).itemsize
+    cache = int(0.5 * (len(h[0]) + len(h[2])) * tb)
+    got, t = fij.ooc_join(dtype, *cols, 11, 3, cache_bytes=cache, staging_tuples=1 << 17)
+    assert got == want
+    assert 0 < t["cached_probe_tuples"] < len(h[2])
+    assert fij.device_status() == 0
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
 @pytest.mark.parametrize("bits", [(9, 6), (4, 3)])
 def test_cpu_pass1(dtype, bits):
     h = make(200_000, 900_001, dtype, 8, dups=True)
@@ -56,6 +71,33 @@ def test_cpu_pass1(dtype, bits):
     got, t = fij.ooc_join(dtype, *h, *bits, pass1_cpu=True, cpu_threads=4, staging_tuples=1 << 17)
     assert got == want
     assert t["cached_build_tuples"] == 0
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("slices", [1, 3, 8])
+@pytest.mark.parametrize("cache_frac", [0.0, 0.5, 2.0])
+def test_cpu_pass1_overlapped_slices(dtype, slices, cache_frac):
+    """CPU pass 1 overlapped with the GPU (cpu_partitioned_radix_join.rs:
+    457-547): the build relation cached in device memory up to cache_bytes
+    while the probe relation is partitioned in slices, every slice joined
+    while the next is partitioned.  A heavy probe key makes one slice's
+    partition longer than the staging groups, so the group buffers grow."""
+    h = make(150_000, 1_000_003, dtype, 13, dups=True)
+    h[2][::7] = 12345  # a seventh of the probe keys
+    want = orc.sort_merge_join(*h)
+    tb = 2 * np.dtype(dtype).itemsize
+    cache = int(cache_frac * len(h[0]) * tb)
+    got, t = fij.ooc_join(dtype, *h, 7, 5, pass1_cpu=True, cpu_threads=4, staging_tuples=1 << 16,
+                          cache_bytes=cache, probe_slices=slices)
+    assert got == want
+    if cache_frac == 2.0:
+        assert t["cached_build_tuples"] == len(h[0]) + 128 * (128 // tb)
+    elif cache_frac == 0.5:
+        assert 0 < t["cached_build_tuples"] < len(h[0])
+    else:
+        assert t["cached_build_tuples"] == 0
+    assert t["groups"] >= slices
+    assert fij.device_status() == 0
 
 
 def test_device_inputs_and_plain_plan():

@@ -136,6 +136,26 @@ def test_partition_unaligned_columns(dtype, hist, shift):
     run_partition(300_007, dtype, hist, fij.RadixPartitionAlgorithm.SSWWCv2, 9, grid=37, block=1024, shift=shift)
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("hist", [CHUNKED, CONTIG])
+@pytest.mark.parametrize("bits", [10, 12, 14])
+@pytest.mark.parametrize("algo", [fij.RadixPartitionAlgorithm.SSWWCv2, fij.RadixPartitionAlgorithm.HSSWWCv4])
+def test_partition_high_fanout_buffered(dtype, hist, bits, algo):
+    """Fanouts whose line rings do not fit LDS: the buffered variants sort
+    each tile of the chunk by digit in LDS and write runs (32 chunks)."""
+    run_partition(4_000_037, dtype, hist, algo, bits, grid=32, block=1024)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("tiles,extra", [(0, 1), (0, 4095), (1, 0), (1, 1), (2, 7), (3, -1)])
+def test_partition_high_fanout_tile_counts(dtype, tiles, extra):
+    """One chunk of every kind of remainder of the sorting scatter's tiles
+    (8192 tuples at i32, 4096 at i64), at 11 bits."""
+    tile = 8192 if dtype == np.int32 else 4096
+    n = tiles * tile + (extra if extra >= 0 else tile + extra)
+    run_partition(n, dtype, CHUNKED, fij.RadixPartitionAlgorithm.SSWWCv2, 11, grid=1, block=1024)
+
+
 @pytest.mark.parametrize("hist", [CHUNKED, CONTIG])
 def test_partition_empty(hist):
     run_partition(0, np.int64, hist, fij.RadixPartitionAlgorithm.SSWWCv2, 4, grid=3, block=256)
