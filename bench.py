@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="served: HBM budget (GB)")
     ap.add_argument("--cache-gb", type=float, default=2.0, help="ooc: device cache for pass-1 partitions (GB)")
     ap.add_argument("--pass1-cpu", action="store_true", help="ooc: pass 1 on host threads")
+    ap.add_argument("--probe-slices", type=int, default=0,
+                    help="ooc with --pass1-cpu: probe-side slices partitioned while the GPU joins the previous "
+                         "one (0 = automatic)")
     ap.add_argument("--plain-sample", type=int, default=0,
                     help="batched: time the plain per-join launch on every k-th join only (0 = all)")
     return ap.parse_args()
@@ -502,7 +505,9 @@ def ooc_main(a):
     (the C2 generators), pass-1 partitions cached in device memory up to
     --cache-gb and spilled to pinned host memory beyond it, pass 2 and the
     join streamed over partition groups.  Pass 1 on the GPU (reading the
-    relations over the host link) or, with --pass1-cpu, on host threads."""
+    relations over the host link) or, with --pass1-cpu, on host threads:
+    then R's cached prefix is copied while the probe side is partitioned in
+    slices, each slice joined on the GPU while the next is partitioned."""
     import fij
 
     n = 1 << a.log2n
@@ -522,7 +527,8 @@ def ooc_main(a):
     fij.synchronize()
     cache = int(a.cache_gb * 1e9)
     threads = a.cpu_threads or min(16, os.cpu_count() or 1)  # the box's CPU share
-    kw = dict(pass1_cpu=a.pass1_cpu, cache_bytes=cache, staging_tuples=1 << 24, cpu_threads=threads)
+    kw = dict(pass1_cpu=a.pass1_cpu, cache_bytes=cache, staging_tuples=1 << 24, cpu_threads=threads,
+              probe_slices=a.probe_slices)
     for _ in range(a.warmup):
         fij.ooc_join(np.int64, *cols, a.bits1, a.bits2, **kw)
     times, walls = [], []
@@ -542,7 +548,8 @@ def ooc_main(a):
                                f"{a.bits1}+{a.bits2} bits, {a.cache_gb} GB device cache, pass 1 on "
                                f"{'host threads' if a.pass1_cpu else 'the GPU'}",
                    "cache_gb": a.cache_gb, "pass1": "cpu" if a.pass1_cpu else "gpu",
-                   "cpu_threads": threads if a.pass1_cpu else 0, "parallelism": "single GPU"},
+                   "cpu_threads": threads if a.pass1_cpu else 0, "probe_slices": a.probe_slices,
+                   "parallelism": "single GPU"},
         "phases_ms": {k: float(np.median([t[k] for t in times])) for k in ("prefix_sum_ms", "partition_ms", "join_ms")},
         "cached_tuples": [times[0]["cached_build_tuples"], times[0]["cached_probe_tuples"]],
         "groups": times[0]["groups"],

@@ -68,7 +68,10 @@ __global__ __launch_bounds__(1024) void plan_kernel(PipelineDevice d) {
     s.tile_start[F1] = static_cast<uint32_t>(total_tiles);
   }
   if (blockIdx.x == 0) {
-    if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
+    if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) {
+      d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
+      d.work_counter[threadIdx.x * kJoinCounterStride + 1] = 0;  // the high half of a 64-bit counter
+    }
     for (uint32_t i = threadIdx.x; i < d.agg_len; i += blockDim.x) {
       d.aggregates[i].count = 0;
       d.aggregates[i].sum_build = 0;
@@ -117,7 +120,10 @@ __global__ __launch_bounds__(1024) void plan_batched_kernel(PipelineDevice d, co
       d.build.pstart[d.F1] = rel2_total_b;
       d.probe.pstart[d.F1] = rel2_total_p;
     }
-    if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
+    if (threadIdx.x < kJoinCounterWords / kJoinCounterStride) {
+      d.work_counter[threadIdx.x * kJoinCounterStride] = 0;
+      d.work_counter[threadIdx.x * kJoinCounterStride + 1] = 0;  // the high half of a 64-bit counter
+    }
   }
   if (side == 0 && threadIdx.x == 0) {
     JoinAggregate* r = d.job_results + d.job_slot[js >> 1];
@@ -726,26 +732,58 @@ __global__ __launch_bounds__(kUnitsWaves * 64) void join_units_kernel(PipelineDe
       btuples += part[1][i][lane];
     }
     if (flagged) d.overflow[item] = 0;
-    const uint32_t path = btuples <= block_rcap ? 0u : 1u;
-    uint32_t units = 0;
+    uint32_t path = btuples <= block_rcap ? 0u : 1u;
+    // A larger build side gets a hash table in the HBM arena when the arena
+    // has room for it (at least twice its tuples, a power of two): path 2.
+    uint32_t big_idx = 0, tslot = 0, tlg = 0;
+    if (flagged && path == 1 && d.hbm_cap && btuples <= (d.hbm_cap >> 1)) {
+      tlg = max(8u, 64u - static_cast<uint32_t>(__builtin_clzll(2 * btuples - 1)));
+      const uint64_t size = 1ull << tlg;
+      const uint64_t first =
+          atomicAdd(reinterpret_cast<unsigned long long*>(d.work_counter + kHbmSlots), static_cast<unsigned long long>(size));
+      if (first + size <= d.hbm_cap) {
+        big_idx = atomicAdd(d.work_counter + kBigCount, 1u);
+        if (big_idx < d.big_cap) {
+          path = 2;
+          tslot = static_cast<uint32_t>(first);
+          d.big[big_idx] = make_uint4(item, tslot, tlg, 0u);
+        }
+      }
+    }
+    uint32_t units = 0, bunits = 0;
     if (flagged) units = max(1u, static_cast<uint32_t>(min<uint64_t>((tuples + kUnitTuples - 1) / kUnitTuples, np)));
+    if (path == 2) bunits = max(1u, static_cast<uint32_t>(min<uint64_t>((btuples + kUnitTuples - 1) / kUnitTuples, nb)));
     // block-table units go to the front of `units`, wave-table units to the
     // back (unit_cap bounds both together), so each join kernel visits only
-    // its own
-    const uint32_t ub = path == 0 ? units : 0u, uw = path == 0 ? 0u : units;
+    // its own; HBM-table build units to the front of `hbm_units`, their probe
+    // units to its back
+    const uint32_t ub = path == 0 ? units : 0u, uw = path == 1 ? units : 0u;
+    const uint32_t hb = path == 2 ? bunits : 0u, hp = path == 2 ? units : 0u;
     const uint32_t incl_b = wave_inclusive_scan(ub), incl_w = wave_inclusive_scan(uw);
-    uint32_t base_b = 0, base_w = 0;
+    const uint32_t incl_hb = wave_inclusive_scan(hb), incl_hp = wave_inclusive_scan(hp);
+    uint32_t base_b = 0, base_w = 0, base_hb = 0, base_hp = 0;
     if (lane == 63) {
       if (incl_b) base_b = atomicAdd(d.work_counter + kUnitCount, incl_b);
       if (incl_w) base_w = atomicAdd(d.work_counter + kWaveUnitCount, incl_w);
+      if (incl_hb) base_hb = atomicAdd(d.work_counter + kHbmBuildUnits, incl_hb);
+      if (incl_hp) base_hp = atomicAdd(d.work_counter + kHbmProbeUnits, incl_hp);
     }
     base_b = __builtin_amdgcn_readlane(base_b, 63) + incl_b - ub;
     base_w = __builtin_amdgcn_readlane(base_w, 63) + incl_w - uw;
+    base_hb = __builtin_amdgcn_readlane(base_hb, 63) + incl_hb - hb;
+    base_hp = __builtin_amdgcn_readlane(base_hp, 63) + incl_hp - hp;
     for (uint32_t u = 0; u < units; ++u) {
       const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(u) * np / units);
       const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(u + 1) * np / units);
-      const uint32_t slot = path == 0 ? base_b + u : d.unit_cap - 1 - (base_w + u);
-      d.units[slot] = make_uint4(t * 64 + lane, r0, r1, path);
+      if (path == 2)
+        d.hbm_units[d.hbm_unit_cap - 1 - (base_hp + u)] = make_uint4(big_idx, r0, r1, 0u);
+      else
+        d.units[path == 0 ? base_b + u : d.unit_cap - 1 - (base_w + u)] = make_uint4(t * 64 + lane, r0, r1, path);
+    }
+    for (uint32_t u = 0; u < bunits; ++u) {
+      const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(u) * nb / bunits);
+      const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(u + 1) * nb / bunits);
+      d.hbm_units[base_hb + u] = make_uint4(big_idx, r0, r1, 0u);
     }
   }
 }
@@ -983,6 +1021,132 @@ __global__ __launch_bounds__(BLOCK) void join_block_kernel(PipelineDevice d) {
     }
   }
   if (!d.job_results) join_block_aggregate<BLOCK / 64>(d, cnt, sb, sp);
+}
+
+// ----------------------------------------------------- HBM hash tables ----
+// A deferred item whose build side exceeds a block's LDS table gets a
+// linear-probing table of 2^lg slots (at least twice its build tuples) in
+// the plan's HBM arena: its build units insert the build side once, then its
+// probe units probe it, so the cost is linear in both sides (the LDS path
+// refills a wave table once per RCAP build tuples and streams the unit's
+// probe runs against every fill).  Hashing and probing follow the
+// no-partitioning join (no_partitioning_join.cu:39-249): multiply-shift from
+// the key, linear probing, every match counted; i32 entries are installed
+// by one 64-bit CAS of key and payload against an all-ones entry, i64 by a
+// CAS on the key and then the payload store.  The arena is all ones between
+// joins: the clear kernel restores the slots a join reserved.
+
+template <typename K>
+__device__ __forceinline__ void hbm_insert(Tuple<K>* tab, uint32_t lg, K key, K val) {
+  const uint64_t mask = (1ull << lg) - 1;
+  uint64_t h = mult_shift_hash<K>(key, lg);
+  for (uint64_t a = 0; a <= mask; ++a, h = (h + 1) & mask) {
+    if constexpr (sizeof(K) == 4) {
+      const unsigned long long want = static_cast<unsigned long long>(static_cast<uint32_t>(key)) |
+                                      (static_cast<unsigned long long>(static_cast<uint32_t>(val)) << 32);
+      if (atomicCAS(reinterpret_cast<unsigned long long*>(tab + h), ~0ull, want) == ~0ull) return;
+    } else {
+      if (atomicCAS(reinterpret_cast<unsigned long long*>(&tab[h].key), ~0ull, static_cast<unsigned long long>(key)) ==
+          ~0ull) {
+        tab[h].value = val;
+        return;
+      }
+    }
+  }
+}
+
+// Every build tuple with this key matches; probing ends at the first free
+// slot.  (The i64 payload of a slot whose key was just installed may still
+// be in flight only while the build kernel runs; probes run in a later
+// launch.)
+template <typename K>
+__device__ __forceinline__ void hbm_probe(const Tuple<K>* tab, uint32_t lg, K key, K pay, uint64_t& cnt, uint64_t& sb,
+                                          uint64_t& sp) {
+  const uint64_t mask = (1ull << lg) - 1;
+  uint64_t h = mult_shift_hash<K>(key, lg);
+  for (uint64_t a = 0; a <= mask; ++a, h = (h + 1) & mask) {
+    const Tuple<K> e = load_tuple<K>(tab + h);
+    if (e.key == null_key<K>()) return;
+    if (e.key == key) {
+      cnt += 1;
+      sb += static_cast<uint64_t>(static_cast<int64_t>(e.value));
+      sp += static_cast<uint64_t>(static_cast<int64_t>(pay));
+    }
+  }
+}
+
+// One block per HBM work unit (tickets over build units when BUILD, else over
+// probe units): the unit's runs of its item's build (probe) side, published
+// block-wide in batches of BLOCK runs, SPL gathers in flight per thread.
+template <typename K, uint32_t BLOCK, uint32_t SPL, bool BUILD>
+__global__ __launch_bounds__(BLOCK) void join_hbm_kernel(PipelineDevice d) {
+  using T = Tuple<K>;
+  __shared__ uint32_t s_unit;
+  __shared__ uint32_t wave_sums[BLOCK / 64];
+  __shared__ uint64_t run_start[BLOCK];
+  __shared__ uint32_t run_prefix[BLOCK];
+  __shared__ uint16_t run_bucket[2 * BLOCK + 4];
+  const BlockRuns br{run_start, run_prefix, run_bucket};
+  const uint32_t tid = threadIdx.x;
+  const uint32_t F2 = d.bits2 ? d.F2 : 1;
+  const PipelineSide& side = BUILD ? d.build : d.probe;
+  const T* rel = static_cast<const T*>(d.bits2 ? side.rel2 : side.rel1);
+  T* const arena = static_cast<T*>(d.hbm_table);
+  uint64_t cnt = 0, sb = 0, sp = 0;
+  const uint32_t n = uniform_load(d.work_counter + kOverflowAny)
+                         ? uniform_load(d.work_counter + (BUILD ? kHbmBuildUnits : kHbmProbeUnits))
+                         : 0u;
+  for (uint32_t u = 0; n;) {
+    if (tid == 0) s_unit = atomicAdd(d.work_counter + (BUILD ? kHbmBuildTicket : kHbmProbeTicket), 1u);
+    __syncthreads();
+    u = s_unit;
+    __syncthreads();  // every thread has read s_unit
+    if (u >= n) break;
+    const uint4 unit = d.hbm_units[BUILD ? u : d.hbm_unit_cap - 1 - u];
+    const uint4 big = d.big[__builtin_amdgcn_readfirstlane(unit.x)];
+    const uint32_t item = __builtin_amdgcn_readfirstlane(big.x);
+    const uint32_t p = item / F2, q = item % F2;
+    T* const tab = arena + __builtin_amdgcn_readfirstlane(big.y);
+    const uint32_t lg = __builtin_amdgcn_readfirstlane(big.z);
+    const uint32_t r_end = __builtin_amdgcn_readfirstlane(unit.z);
+    for (uint32_t r0 = __builtin_amdgcn_readfirstlane(unit.y); r0 < r_end; r0 += BLOCK) {
+      const uint32_t tn = publish_block_runs<BLOCK>(d, side, p, q, r0, r_end, br, wave_sums);
+      const uint32_t s0 = publish_block_buckets<BLOCK>(br, min(r_end - r0, BLOCK), tn);
+      for (uint32_t e0 = 0; e0 < tn; e0 += SPL * BLOCK) {
+        uint32_t e[SPL];
+        uint64_t idx[SPL];
+        T t[SPL];
+#pragma unroll
+        for (uint32_t k = 0; k < SPL; ++k) e[k] = min(e0 + k * BLOCK + tid, tn - 1);
+        block_elements(br, e, s0, idx);
+#pragma unroll
+        for (uint32_t k = 0; k < SPL; ++k) t[k] = load_tuple<K>(rel + idx[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < SPL; ++k) {
+          if (e0 + k * BLOCK + tid >= tn || t[k].key == null_key<K>()) continue;  // null keys never join
+          if constexpr (BUILD)
+            hbm_insert<K>(tab, lg, t[k].key, t[k].value);
+          else
+            hbm_probe<K>(tab, lg, t[k].key, t[k].value, cnt, sb, sp);
+        }
+      }
+      __syncthreads();  // the batch may be overwritten
+    }
+    if (!BUILD && d.job_results) wave_job_aggregate(d, p >> d.job_shift, cnt, sb, sp);
+  }
+  if (!BUILD && !d.job_results) join_block_aggregate<BLOCK / 64>(d, cnt, sb, sp);
+}
+
+// Restores the slots reserved in this join to all ones (16-byte stores).
+__global__ __launch_bounds__(256) void join_hbm_clear_kernel(PipelineDevice d, uint32_t tuple_bytes) {
+  if (uniform_load(d.work_counter + kOverflowAny) == 0) return;
+  const uint64_t reserved = *reinterpret_cast<const volatile uint64_t*>(d.work_counter + kHbmSlots);
+  const uint64_t slots = min<uint64_t>(reserved, d.hbm_cap);
+  uint4* w = static_cast<uint4*>(d.hbm_table);
+  const uint64_t n16 = slots * tuple_bytes / 16;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n16;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    w[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
 }
 
 // One wave per work item (p, q).  Items are handed out in tickets of up to 8
@@ -1270,6 +1434,15 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(gen), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds));
   hipLaunchKernelGGL(gen, dim3(grid), dim3(GWAVES * 64), lds, stream, d);
+  e = hipGetLastError();
+  if (e != hipSuccess || d.hbm_cap == 0) return e;
+  // items with an HBM table: build, probe, then restore the arena
+  hipLaunchKernelGGL((join_hbm_kernel<K, 512, 8, true>), dim3(grid), dim3(512), 0, stream, d);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL((join_hbm_kernel<K, 512, 8, false>), dim3(grid), dim3(512), 0, stream, d);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(join_hbm_clear_kernel, dim3(grid), dim3(256), 0, stream, d,
+                     static_cast<uint32_t>(sizeof(Tuple<K>)));
   return hipGetLastError();
 }
 

@@ -40,7 +40,9 @@ struct PipelineSide {
 // deferred, the number of block-table work units made of them (stored from
 // the front of `units`), the ticket counter of the wave-table units, that of
 // the block-table units, and the number of wave-table units (stored from the
-// back of `units`).
+// back of `units`); then the HBM-table counters: slots reserved in the HBM
+// table arena (a 64-bit word), items given an HBM table, their build and
+// probe work units, and the tickets over those.
 constexpr uint32_t kJoinCounters = 8;
 constexpr uint32_t kJoinCounterStride = 32;
 constexpr uint32_t kOverflowTicket = kJoinCounters * kJoinCounterStride;
@@ -49,7 +51,13 @@ constexpr uint32_t kUnitCount = (kJoinCounters + 2) * kJoinCounterStride;
 constexpr uint32_t kUnitTicket = (kJoinCounters + 3) * kJoinCounterStride;
 constexpr uint32_t kBlockTicket = (kJoinCounters + 4) * kJoinCounterStride;
 constexpr uint32_t kWaveUnitCount = (kJoinCounters + 5) * kJoinCounterStride;
-constexpr uint32_t kJoinCounterWords = (kJoinCounters + 6) * kJoinCounterStride;
+constexpr uint32_t kHbmSlots = (kJoinCounters + 6) * kJoinCounterStride;
+constexpr uint32_t kBigCount = (kJoinCounters + 7) * kJoinCounterStride;
+constexpr uint32_t kHbmBuildUnits = (kJoinCounters + 8) * kJoinCounterStride;
+constexpr uint32_t kHbmProbeUnits = (kJoinCounters + 9) * kJoinCounterStride;
+constexpr uint32_t kHbmBuildTicket = (kJoinCounters + 10) * kJoinCounterStride;
+constexpr uint32_t kHbmProbeTicket = (kJoinCounters + 11) * kJoinCounterStride;
+constexpr uint32_t kJoinCounterWords = (kJoinCounters + 12) * kJoinCounterStride;
 
 struct PipelineDevice {
   PipelineSide build;
@@ -72,6 +80,15 @@ struct PipelineDevice {
   JoinAggregate* job_results;
   const uint32_t* job_slot;
   uint32_t job_shift;
+  // HBM hash tables of deferred items whose build side exceeds a block's
+  // LDS table.  hbm_cap == 0: none; such items refill wave LDS tables.
+  void* hbm_table;            // Tuple[hbm_cap], every word all ones between joins
+  uint32_t hbm_cap;           // slots (power of two)
+  uint32_t big_cap;           // entries of `big`
+  uint4* big;                 // [big_cap] {item, first slot, log2 slots, 0}
+  uint4* hbm_units;           // [hbm_unit_cap] {big index, first run, end run, 0}: build units from the front,
+                              // probe units from the back
+  uint32_t hbm_unit_cap;
 };
 
 // One join side of a batched plan: its pass-1 region and where its

@@ -166,6 +166,27 @@ fij_status fij_join_plan_create(const fij_join_plan_config* cfg, fij_join_plan**
   // them); every join leaves them at zero
   if (st == FIJ_OK && (hipMemset(p->d.overflow, 0, items) != hipSuccess || hipDeviceSynchronize() != hipSuccess))
     st = plan_fail(FIJ_DEVICE_ERROR, "clearing the deferred-item flags failed");
+  // HBM hash tables of deferred items with large build sides: an arena of
+  // 2^k slots, twice the build relation (at most 2^26 slots), every word all
+  // ones; one table per item at most, each at least 256 slots
+  {
+    uint64_t slots = 1024;
+    while (slots < 2 * cfg->build_len && slots < (1ull << 26)) slots <<= 1;
+    p->d.hbm_cap = static_cast<uint32_t>(slots);
+    p->d.big_cap = static_cast<uint32_t>(std::min<uint64_t>(items, slots / 256));
+    p->d.hbm_unit_cap =
+        static_cast<uint32_t>(2ull * p->d.big_cap + (cfg->build_len + cfg->probe_len) / 65536 + 2);
+    void* arena = nullptr;
+    if (st == FIJ_OK && hipMalloc(&arena, slots * tuple_bytes(kt)) != hipSuccess)
+      st = plan_fail(FIJ_OUT_OF_MEMORY, "cannot allocate the HBM hash-table arena");
+    if (arena) p->allocs.push_back(arena);
+    p->d.hbm_table = arena;
+    if (st == FIJ_OK) st = alloc(p, &p->d.big, p->d.big_cap);
+    if (st == FIJ_OK) st = alloc(p, &p->d.hbm_units, p->d.hbm_unit_cap);
+    if (st == FIJ_OK && (hipMemset(arena, 0xFF, slots * tuple_bytes(kt)) != hipSuccess ||
+                         hipDeviceSynchronize() != hipSuccess))
+      st = plan_fail(FIJ_DEVICE_ERROR, "initialising the HBM hash-table arena failed");
+  }
   if (st == FIJ_OK) st = alloc(p, &p->d.aggregates, p->join_grid);
   for (int i = 0; i < 6 && st == FIJ_OK; ++i)
     if (hipEventCreate(&p->ev[i]) != hipSuccess) st = plan_fail(FIJ_DEVICE_ERROR, "hipEventCreate failed");
@@ -323,6 +344,7 @@ fij_status fij_join_plan_device_bytes(const fij_join_plan* p, uint64_t* bytes) {
     if (p->d.bits2) total += s->n * tb;
   }
   if (p->d.bits2) total += (static_cast<uint64_t>(p->max_tiles[0]) + p->max_tiles[1] + 2) * p->d.F2 * 4;
+  total += static_cast<uint64_t>(p->d.hbm_cap) * tb + (static_cast<uint64_t>(p->d.big_cap) + p->d.hbm_unit_cap) * 16;
   *bytes = total;
   return FIJ_OK;
 }

@@ -47,8 +47,11 @@ def gpu_table(keys, pays, entries, dtype, dims=(64, 256)):
 
 
 def oracle_table(keys, pays, entries, dtype):
+    # every 32-bit word all ones, as HashTable::new_on_gpu initialises device
+    # tables (no_partitioning_join.rs:611-620, cuMemsetD32 of the null key)
     table = np.zeros(entries, orc.tuple_dtype(dtype))
     table["key"] = -1
+    table["value"] = -1
     getattr(orc.o, f"orc_np_build_lp_{orc.suffix(dtype)}")(orc.P(table), entries, orc.P(keys), orc.P(pays),
                                                           len(keys))
     return table["key"].copy(), table["value"].copy()

@@ -79,3 +79,29 @@ def test_heavy_build_key(dtype):
     want = orc.sort_merge_join(rk, rp, sk, sp)
     for bits in ((9, 9), (7, 0)):
         assert run_plan(rk, rp, sk, sp, *bits) == want
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("bits", [(9, 9), (6, 0)])
+def test_large_build_partitions_in_hbm_tables(dtype, bits):
+    """Build keys repeated far beyond a block's LDS table (40,000 and 15,000
+    copies of two keys, and a third key whose copies miss every probe): their
+    items get hash tables in the plan's HBM arena, built once and probed by
+    many units.  The same plan joins three different inputs in a row, so the
+    arena must be back to all ones after every join."""
+    n_r, n_s = 1 << 18, 1 << 21
+    plan = fij.JoinPlan(dtype, n_r, n_s, *bits)
+    for rep in range(3):
+        rk = orc.gen_uniform(n_r, 40 + rep, 1, 1, 1 << 17, dtype)
+        rk[:40_000] = 777 + rep
+        rk[40_000:55_000] = 90_001
+        rk[55_000:70_000] = 99_999_999  # never probed
+        rp = orc.gen_uniform(n_r, 40 + rep, 2, -(1 << 20), 1 << 21, dtype)
+        sk = orc.gen_uniform(n_s, 40 + rep, 3, 1, 1 << 17, dtype)
+        sk[::1009] = 777 + rep
+        sk[5::2003] = 90_001
+        sp = orc.gen_iota(n_s, rep, dtype)
+        got, t = plan.execute(to_dev(rk), to_dev(rp), to_dev(sk), to_dev(sp))
+        assert fij.device_status() == 0
+        assert got == orc.sort_merge_join(rk, rp, sk, sp), rep
+        assert got[0] > 40_000 * (n_s // 1009)
