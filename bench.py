@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--probe-slices", type=int, default=0,
                     help="ooc with --pass1-cpu: probe-side slices partitioned while the GPU joins the previous "
                          "one (0 = automatic)")
+    ap.add_argument("--lanes", type=int, default=4, help="batched: streams the size classes are spread over")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched: skip the per-join comparators (a profile of the batched form alone)")
     ap.add_argument("--plain-sample", type=int, default=0,
                     help="batched: time the plain per-join launch on every k-th join only (0 = all)")
     return ap.parse_args()
@@ -243,7 +246,7 @@ def batched_main(a):
         cur.append(i)
         acc += per_join_bytes[i]
     waves.append(cur)
-    batch = fij.JoinBatch(np.int32, max(len(w) for w in waves), use_graph=True)
+    batch = fij.JoinBatch(np.int32, max(len(w) for w in waves), use_graph=True, lanes=a.lanes)
     res_dev = fij.DeviceArray(3, np.uint64)
     # per-join algorithmic bytes: a small join (one block, LDS table) reads
     # each tuple's key and payload once (t); a partitioned join with P passes
@@ -292,7 +295,7 @@ def batched_main(a):
         for (r, s_), g in zip(lens, got):
             assert g[0] == s_ and g[2] == s_ * (s_ - 1) // 2, "batched join result"
         # the plain per-join launch, one join at a time, every join
-        for j in range(len(joins)):
+        for j in range(0 if a.no_plain else len(joins)):
             _, b1, b2 = classes[w[j]]
             fij.synchronize()
             t0 = time.perf_counter()
@@ -303,7 +306,7 @@ def batched_main(a):
             assert g == got[j], "plain and batched results differ"
             checked += 1
         # with a plan allocated per join, on a sample
-        for j in range(0, len(joins), plain_step):
+        for j in range(0, 0 if a.no_plain else len(joins), plain_step):
             r, s_ = lens[j]
             _, b1, b2 = classes[w[j]]
             t0 = time.perf_counter()
@@ -323,14 +326,14 @@ def batched_main(a):
         "dtype": "int32", "data": "synthetic (seeded on-device generators)",
         "config": {"workload": f"batched joins: {a.joins} int32 joins, |R| log-uniform 2^10..2^22, "
                                "|S| = |R| x U{4..16}, count + sums per join", "joins": a.joins,
-                   "tuples": total, "waves": len(waves), "parallelism": "single GPU"},
+                   "tuples": total, "waves": len(waves), "lanes": a.lanes, "parallelism": "single GPU"},
         "batched": {"seconds": t_batch, "gpu_seconds": gpu_batch, "tuples_per_s": value, **stats_acc},
         "algorithmic_bytes": b_alg,
         "pipeline_roofline_frac": b_alg / gpu_batch / (HBM_PEAK_GBS * 1e9),
         "wall_roofline_frac": b_alg / t_batch / (HBM_PEAK_GBS * 1e9),
-        "plain_per_join": {"seconds": t_reused, "tuples_per_s": total / t_reused,
+        "plain_per_join": {"seconds": t_reused, "tuples_per_s": total / t_reused if t_reused else None,
                            "note": "one join at a time through a preallocated plan of its size class"},
-        "speedup_vs_plain": t_reused / t_batch,
+        "speedup_vs_plain": t_reused / t_batch if t_reused else None,
         "plain_per_join_allocating": {
             "seconds_per_join": t_alloc / max(n_alloc, 1), "joins_timed": n_alloc,
             "note": "a plan created, executed and destroyed per join (allocation timed), every "
@@ -690,6 +693,17 @@ def operator_main(a):
         block = 512 if alg == fij.RadixPartitionAlgorithm.HSSWWCv4 else 1024
         ms = partitioned(sk, sp, alg, block)[3]
         ops[f"partition_{alg.name}"] = line(ms, n_s * (2 * k + t))
+    # high fanouts: line rings no longer fit LDS, the buffered variants sort
+    # each tile by digit; the direct scatter (NC) beside them
+    for hb in (10, 12, 14):
+        for alg in (fij.RadixPartitionAlgorithm.SSWWCv2, fij.RadixPartitionAlgorithm.NC):
+            pt = fij.GpuRadixPartitioner(CH, alg, fij.RadixBits(hb), cus, 1024)
+            of = fij.PartitionOffsets(CH, cus, hb, kt)
+            rel = fij.PartitionedRelation(n_s, CH, hb, cus, kt)
+            ms = timed(lambda: pt.partition(FIRST, sk, sp, of, rel, stream=st),
+                       before=lambda: pt.prefix_sum(FIRST, sk, of, stream=st))
+            ops[f"partition_{alg.name}_{hb}bits"] = line(ms, n_s * (2 * k + t))
+            del pt, of, rel
     # the join's inputs: contiguous layout (one range per partition), as
     # the join and its task assignment take single-chunk relations
     psum_r = timed(lambda: part.prefix_sum(FIRST, rk, fij.PartitionOffsets(CO, cus, bits, kt), stream=st))

@@ -82,6 +82,9 @@ struct ClassArena {
   uint32_t sub_joins = 0, sub_blocks = 0, sub_tiles[2] = {0, 0};
 };
 
+// Streams a submission's independent size classes are spread over.
+constexpr uint32_t kMaxBatchLanes = 4;
+
 // A size class's configuration as batches run it: radix bits, pass-1 line
 // ring cap (0 = deepest) and pass-1 block shape.
 struct ClassKey {
@@ -113,6 +116,12 @@ struct fij_join_batch {
   hipStream_t own = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::map<ClassKey, ClassArena> arenas;
+  // parallel lanes: lane 0 is the submission stream, lanes 1.. their own
+  // streams, forked from it and joined back by events
+  uint32_t lanes = 1;
+  hipStream_t lane[kMaxBatchLanes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kMaxBatchLanes] = {};
 };
 
 namespace {
@@ -475,6 +484,13 @@ fij_status fij_join_batch_create(const fij_join_batch_config* cfg, fij_join_batc
       hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess)
     st = batch_fail(FIJ_OUT_OF_MEMORY, "cannot allocate the batch's descriptors and results");
+  b->lanes = std::min<uint32_t>(kMaxBatchLanes, std::max<uint32_t>(1, cfg->lanes));
+  if (st == FIJ_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    st = batch_fail(FIJ_DEVICE_ERROR, "cannot create the batch's events");
+  for (uint32_t l = 1; l < b->lanes && st == FIJ_OK; ++l)
+    if (hipStreamCreateWithFlags(&b->lane[l], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming) != hipSuccess)
+      st = batch_fail(FIJ_DEVICE_ERROR, "cannot create the batch's lane streams");
   if (st != FIJ_OK) {
     fij_join_batch_destroy(b);
     return st;
@@ -494,6 +510,14 @@ fij_status fij_join_batch_destroy(fij_join_batch* b) {
   if (b->h_results) (void)hipHostFree(b->h_results);
   for (hipEvent_t e : b->ev)
     if (e) (void)hipEventDestroy(e);
+  for (uint32_t l = 1; l < kMaxBatchLanes; ++l) {
+    if (b->lane[l]) {
+      (void)hipStreamSynchronize(b->lane[l]);
+      (void)hipStreamDestroy(b->lane[l]);
+    }
+    if (b->join[l]) (void)hipEventDestroy(b->join[l]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->own) (void)hipStreamDestroy(b->own);
   delete b;
   return FIJ_OK;
@@ -552,11 +576,39 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
     hipError_t e = launch_small_joins(b->kt, b->d_jobs, n_small, b->d_ticket, b->cus, st);
     if (e != hipSuccess) s = batch_fail(FIJ_DEVICE_ERROR, "small joins: %s", hipGetErrorString(e));
   }
+  // Size classes are independent (each has its own buffers and result
+  // slots), so they are spread over the lanes: largest class first, each to
+  // the lane with the fewest tuples so far.  Lane 0 is the submission stream
+  // (which also runs the small joins); the other lanes wait for it to reach
+  // this point and it waits for all of them at the end.
+  std::vector<std::pair<uint64_t, const ClassKey*>> order;
   for (auto& kv : classes) {
+    uint64_t tuples = 0;
+    for (uint32_t i : kv.second) tuples += joins[i].build_len + joins[i].probe_len;
+    order.emplace_back(tuples, &kv.first);
+  }
+  std::sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+  const uint32_t lanes = std::min<uint32_t>(b->lanes, std::max<uint32_t>(1, static_cast<uint32_t>(order.size())));
+  uint64_t lane_load[kMaxBatchLanes] = {};
+  if (lanes > 1 && s == FIJ_OK) {
+    hipError_t e = hipEventRecord(b->fork, st);
+    for (uint32_t l = 1; l < lanes && e == hipSuccess; ++l) e = hipStreamWaitEvent(b->lane[l], b->fork, 0);
+    if (e != hipSuccess) s = batch_fail(FIJ_DEVICE_ERROR, "forking the lanes: %s", hipGetErrorString(e));
+  }
+  for (const auto& [tuples, key] : order) {
     if (s != FIJ_OK) break;
-    hipError_t e = launch_class(b, b->arenas[kv.first], kv.first, st, b->lds_total);
+    const uint32_t l = static_cast<uint32_t>(std::min_element(lane_load, lane_load + lanes) - lane_load);
+    lane_load[l] += tuples;
+    hipError_t e = launch_class(b, b->arenas[*key], *key, l ? b->lane[l] : st, b->lds_total);
     if (e != hipSuccess)
-      s = batch_fail(FIJ_DEVICE_ERROR, "size class %u+%u: %s", kv.first.b1, kv.first.b2, hipGetErrorString(e));
+      s = batch_fail(FIJ_DEVICE_ERROR, "size class %u+%u: %s", key->b1, key->b2, hipGetErrorString(e));
+  }
+  // every forked lane joins back, also after an error (a capture must end
+  // with every stream it forked joined)
+  for (uint32_t l = 1; l < lanes; ++l) {
+    hipError_t e = hipEventRecord(b->join[l], b->lane[l]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, b->join[l], 0);
+    if (e != hipSuccess && s == FIJ_OK) s = batch_fail(FIJ_DEVICE_ERROR, "joining the lanes: %s", hipGetErrorString(e));
   }
   if (use_graph) {
     hipError_t e = hipStreamEndCapture(st, &graph);

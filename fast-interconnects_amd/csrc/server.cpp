@@ -133,7 +133,7 @@ struct fij_join_server {
   uint32_t* scratch_counters = nullptr;
   JoinAggregate* scratch_agg = nullptr;
   hipStream_t setup = nullptr;
-  std::mutex mu;
+  mutable std::mutex mu;
   std::condition_variable cv;
   std::deque<uint64_t> queue;
   std::map<uint64_t, Request> requests;
@@ -521,6 +521,8 @@ fij_status fij_join_server_wait(fij_join_server* s, uint64_t request_id, fij_joi
 
 fij_status fij_join_server_memory(const fij_join_server* s, uint64_t* used_bytes, uint32_t* builds) {
   if (!s || !used_bytes) return server_fail(FIJ_INVALID_ARGUMENT, "invalid argument");
+  // add_build and the scheduler change both under the lock
+  std::lock_guard<std::mutex> lock(s->mu);
   *used_bytes = s->budget_used;
   if (builds) *builds = static_cast<uint32_t>(s->builds.size());
   return FIJ_OK;

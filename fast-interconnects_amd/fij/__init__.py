@@ -814,7 +814,7 @@ class JoinDesc(C.Structure):
 
 
 class JoinBatchConfig(C.Structure):
-    _fields_ = [("type", _u32), ("max_joins", _u32), ("use_graph", _u32), ("reserved", _u32)]
+    _fields_ = [("type", _u32), ("max_joins", _u32), ("use_graph", _u32), ("lanes", _u32)]
 
 
 class JoinBatchStats(C.Structure):
@@ -874,8 +874,8 @@ class JoinBatch:
     (build_keys, build_payloads, probe_keys, probe_payloads) DeviceArrays and
     returns one (count, sum_build, sum_probe) per join and the stats."""
 
-    def __init__(self, dtype, max_joins, use_graph=True):
-        cfg = JoinBatchConfig(int(_ktype(dtype)), max_joins, 1 if use_graph else 0, 0)
+    def __init__(self, dtype, max_joins, use_graph=True, lanes=4):
+        cfg = JoinBatchConfig(int(_ktype(dtype)), max_joins, 1 if use_graph else 0, lanes)
         h = _vp()
         check(lib.fij_join_batch_create(C.byref(cfg), C.byref(h)))
         self.h = h.value

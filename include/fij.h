@@ -50,8 +50,10 @@ typedef enum fij_histogram_algorithm {
 
 /* sql-ops/src/partition/gpu_radix_partition.rs:241-293 (GpuRadixPartitionAlgorithm).
  * NC is the direct scatter; the four write-combining variants all select
- * this library's LDS line-buffered scatter (falling back to the direct
- * scatter when the fanout's buffers exceed LDS). */
+ * this library's LDS line-buffered scatter, and where the fanout's line
+ * rings exceed LDS (10 or more bits) its tile-sorting scatter, which sorts
+ * each tile of the chunk by digit in LDS and writes runs; only where not even
+ * a tile and its counters fit LDS does the direct scatter run. */
 typedef enum fij_partition_algorithm {
   FIJ_PARTITION_NC = 0,
   FIJ_PARTITION_LASWWC = 1,
@@ -375,7 +377,7 @@ typedef struct fij_join_batch_config {
   uint32_t type;      /* fij_type */
   uint32_t max_joins; /* joins per execute call */
   uint32_t use_graph; /* record the submission into a HIP graph */
-  uint32_t reserved;
+  uint32_t lanes;     /* streams the independent size classes are spread over (0 or 1: one; at most 4) */
 } fij_join_batch_config;
 
 typedef struct fij_join_batch_stats {

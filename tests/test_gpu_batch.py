@@ -21,16 +21,18 @@ def make_join(i, n_r, factor, dtype, seed=1000):
     return rk, rp, sk, sp
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
 @pytest.mark.parametrize("use_graph", [True, False])
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
-def test_batch_matches_each_join_alone(dtype, use_graph):
+def test_batch_matches_each_join_alone(dtype, use_graph, lanes):
     rng = np.random.default_rng(7)
     sizes = [int(2 ** rng.uniform(6, 18)) for _ in range(40)] + [0, 1, 5000, 12288, 12289, 1 << 17]
     host = [make_join(i, max(n, 0), int(rng.integers(4, 17)), dtype) for i, n in enumerate(sizes)]
     dev = [tuple(to_dev(a) if len(a) else fij.DeviceArray(1, dtype) for a in j) for j in host]
     # empty columns: pass zero-length views of real buffers
     dev = [tuple(d if len(h) else d.view(0, 0) for d, h in zip(dj, hj)) for dj, hj in zip(dev, host)]
-    batch = fij.JoinBatch(dtype, len(host), use_graph=use_graph)
+    # lanes: the size classes spread over one or four streams
+    batch = fij.JoinBatch(dtype, len(host), use_graph=use_graph, lanes=lanes)
     for _ in range(2):  # the second execution reuses the pooled plans
         got, stats = batch.execute(dev)
         for j, (h, g) in enumerate(zip(host, got)):

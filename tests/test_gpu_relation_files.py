@@ -22,7 +22,9 @@ from gpu_helpers import sync, to_dev
 
 pytestmark = pytest.mark.gpu
 
-CHUNKED = fij.HistogramAlgorithm.Chunked
+# the radix join takes single-range (contiguous) relations, as the
+# reference's (cuda_radix_join.rs:183-193)
+CONTIG = fij.HistogramAlgorithm.Contiguous
 
 
 def write_generator_file(path, keys, pays, gz=False):
@@ -47,9 +49,9 @@ def operator_join(rk, rp, sk, sp, bits):
     kt = fij.KeyType.I32 if rk.dtype == np.int32 else fij.KeyType.I64
     rels = []
     for k, p in ((rk, rp), (sk, sp)):
-        part = fij.GpuRadixPartitioner(CHUNKED, fij.RadixPartitionAlgorithm.SSWWCv2, fij.RadixBits(bits), 8, 256)
-        offs = fij.PartitionOffsets(CHUNKED, 8, bits, kt)
-        rel = fij.PartitionedRelation(len(k), CHUNKED, bits, 8, kt)
+        part = fij.GpuRadixPartitioner(CONTIG, fij.RadixPartitionAlgorithm.SSWWCv2, fij.RadixBits(bits), 8, 256)
+        offs = fij.PartitionOffsets(CONTIG, 8, bits, kt)
+        rel = fij.PartitionedRelation(len(k), CONTIG, bits, 8, kt)
         dk, dp = to_dev(k), to_dev(p)
         part.prefix_sum(fij.RadixPass.First, dk, offs)
         part.partition(fij.RadixPass.First, dk, dp, offs, rel)
