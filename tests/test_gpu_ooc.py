@@ -8,7 +8,7 @@ import pytest
 
 import fij
 import oracle_lib as orc
-from gpu_helpers import sync, to_dev
+from gpu_helpers import assert_same_partitions, relation_ranges, sync, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -130,14 +130,21 @@ def test_empty_relations():
 @pytest.mark.parametrize("hist", [0, 1])
 def test_cpu_histogram_with_gpu_partitioner(dtype, hist):
     """Offsets from the host prefix sum, copied into device offsets, drive
-    the GPU scatter to the same relation layout as the GPU prefix sum."""
+    the GPU scatter: both the host-driven and the device-driven relation
+    equal the oracle's prefix sum and partition (offsets bit for bit, every
+    (chunk, partition) the same tuple multiset, padding untouched)."""
     n, bits, grid = 1_000_003, 9, 40
     keys = orc.gen_uniform(n, 2, 3, 0, 1 << 30, dtype)
     pays = orc.gen_iota(n, 0, dtype)
     kt = fij.KeyType.I32 if dtype == np.int32 else fij.KeyType.I64
     dk, dp = to_dev(keys), to_dev(pays)
     algo = fij.RadixPartitionAlgorithm.SSWWCv2
-    rels = []
+    if hist == 0:
+        ref_off = orc.chunked_prefix_sum(keys, grid, bits)
+        chunk_off, chunks = ref_off, grid
+    else:
+        ref_off, ref_local = orc.contiguous_prefix_sum(keys, grid, bits)
+        chunk_off, chunks = ref_local, 1
     for on_cpu in (False, True):
         part = fij.GpuRadixPartitioner(hist, algo, fij.RadixBits(bits), grid, 1024)
         offs = fij.PartitionOffsets(hist, grid, bits, kt)
@@ -145,31 +152,20 @@ def test_cpu_histogram_with_gpu_partitioner(dtype, hist):
         if on_cpu:
             host = fij.cpu_prefix_sum(hist, keys, bits, grid)
             if hist == 1:
+                np.testing.assert_array_equal(host[0], ref_off)
+                np.testing.assert_array_equal(host[1], ref_local)
                 offs.offsets.copy_from_host(host[0])
                 offs.local_offsets.copy_from_host(host[1])
             else:
+                np.testing.assert_array_equal(host, ref_off)
                 offs.offsets.copy_from_host(host)
             offs.s.data_len = n
         else:
             part.prefix_sum(fij.RadixPass.First, dk, offs)
         part.partition(fij.RadixPass.First, dk, dp, offs, rel)
         sync()
-        rels.append((rel.host_offsets(), *rel.host_relation()))
-    (o0, k0, v0), (o1, k1, v1) = rels
-    np.testing.assert_array_equal(o0, o1)
-    # same partitions, same tuples (order within a partition is not defined)
-    ranges_chunks = grid if hist == 0 else 1
-    pad = orc.pad_len(dtype)
-    F = 1 << bits
-    starts = o0.reshape(ranges_chunks, F) if hist == 0 else o0.reshape(1, F)
-    ends = []
-    for c in range(ranges_chunks):
-        for p in range(F):
-            b = int(starts[c, p])
-            nxt = int(starts[c, p + 1]) - pad if p + 1 < F else (
-                int(starts[c + 1, 0]) - pad if c + 1 < ranges_chunks else len(k0))
-            ends.append((b, nxt))
-    for b, e in ends:
-        a = np.sort(k0[b:e].astype(np.int64) * 2_000_003 + v0[b:e])
-        c = np.sort(k1[b:e].astype(np.int64) * 2_000_003 + v1[b:e])
-        np.testing.assert_array_equal(a, c)
+        np.testing.assert_array_equal(rel.host_offsets(), ref_off)
+        gk, gv = rel.host_relation()
+        ref = orc.partition(keys, pays, grid, bits, 0, chunk_off, rel.s.relation_len)
+        ranges = relation_ranges(ref_off, chunks, bits, rel.padded_len(), orc.pad_len(dtype))
+        assert_same_partitions(gk, gv, ref["key"], ref["value"], ranges)
