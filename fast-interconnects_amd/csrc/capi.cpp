@@ -417,6 +417,41 @@ fij_status fij_prefix_sum(fij_partitioner* p, fij_radix_pass pass, fij_type type
   return FIJ_OK;
 }
 
+// fij_prefix_sum without any device allocation: the contiguous histogram's
+// counters and look-back state come from the library's static scratch array
+// (noalloc_scratch), not from the partitioner's lazily grown buffers.  The
+// device-info set-up (which allocates the status word) is not needed here.
+fij_status fij_prefix_sum_noalloc(fij_partitioner* p, fij_radix_pass pass, fij_type type, const void* partition_attr,
+                                  uint64_t len, fij_partition_offsets* offsets, void* stream) {
+  uint32_t bits = 0;
+  fij_status s = prefix_sum_checks(p, pass, type, partition_attr, len, offsets, &bits);
+  if (s != FIJ_OK) return s;
+  const uint32_t F = 1u << bits;
+  const uint64_t n = static_cast<uint64_t>(F) * p->grid;
+  if (p->hist == FIJ_HISTOGRAM_CONTIGUOUS && n > kNoAllocCounters)
+    return fail(FIJ_INVALID_ARGUMENT,
+                "fanout x grid = %llu exceeds the %llu counters of the allocation-free prefix sum's scratch",
+                static_cast<unsigned long long>(n), static_cast<unsigned long long>(kNoAllocCounters));
+  offsets->data_len = len;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t ignore = pass_ignore_bits(p->bits, pass);
+  if (p->hist == FIJ_HISTOGRAM_CHUNKED) {
+    // needs no scratch: the chunked histogram writes the offsets directly
+    FIJ_HIP(launch_hist_chunked(key_type(type), partition_attr, len, p->grid, bits, ignore, offsets->offsets,
+                                p->block, st));
+    return FIJ_OK;
+  }
+  uint32_t* counts = nullptr;
+  uint64_t* state = nullptr;
+  FIJ_HIP(noalloc_scratch(&counts, &state));
+  // the histogram kernel clears the scan state it needs before the scan
+  FIJ_HIP(launch_hist_counts(key_type(type), partition_attr, nullptr, nullptr, nullptr, len, p->grid, bits, ignore,
+                             counts, state, scan_state_words(n), p->block, st));
+  FIJ_HIP(launch_offsets_scan(counts, F, p->grid, pad_tuples(key_type(type)), offsets->local_offsets,
+                              offsets->offsets, state, st));
+  return FIJ_OK;
+}
+
 fij_status fij_prefix_sum_and_copy_with_payload(fij_partitioner* p, fij_radix_pass pass, fij_type type,
                                                 const void* src_partition_attr, const void* src_payload_attr,
                                                 uint64_t len, void* dst_partition_attr, void* dst_payload_attr,

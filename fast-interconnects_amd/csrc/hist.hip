@@ -6,6 +6,8 @@
 // gfx950-specific: wave64 ballot/mbcnt aggregation into per-wave LDS
 // sub-histograms, DPP wave scans, and a decoupled look-back device scan
 // with a dynamic tile ticket instead of cooperative grid barriers.
+#include <mutex>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -524,6 +526,40 @@ hipError_t launch_offsets_scan(const uint32_t* counts, uint32_t fanout, uint32_t
   hipLaunchKernelGGL(offsets_scan_kernel, dim3(tiles), dim3(kScanBlock), 0, stream, counts, fanout, chunks,
                      pad, local_offsets, offsets, scan_state);
   return hipGetLastError();
+}
+
+// Scratch of the allocation-free contiguous prefix sum: counts[F][chunks]
+// (u32, kNoAllocCounters of them) followed by the look-back scan state
+// (u64, scan_state_words(kNoAllocCounters)).  One copy per device, held until
+// the process ends; every call on a device shares it.
+namespace {
+constexpr uint64_t kNoAllocBytes = kNoAllocCounters * 4 + scan_state_words(kNoAllocCounters) * 8;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+void* g_scratch_ptr[kNoAllocDevices];
+std::mutex g_scratch_mutex;
+}  // namespace
+
+hipError_t noalloc_scratch(uint32_t** counts, uint64_t** scan_state) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kNoAllocDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    // the array's name is a host-side stand-in: the device address of this
+    // device's copy comes from the runtime, with this device current
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = ptr;
+  }
+  unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  *counts = reinterpret_cast<uint32_t*>(base);
+  *scan_state = reinterpret_cast<uint64_t*>(base + kNoAllocCounters * 4);
+  return hipSuccess;
 }
 
 }  // namespace fij

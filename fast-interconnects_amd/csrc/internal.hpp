@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fij.h"
+
 namespace fij {
 
 // Which element type a launch works on (i32 keys + i32 payloads, or i64/i64).
@@ -55,6 +57,13 @@ hipError_t launch_transform_counts(KeyType t, const void* src_rel, const uint64_
 hipError_t launch_offsets_scan(const uint32_t* counts, uint32_t fanout, uint32_t chunks,
                                uint32_t pad, uint64_t* local_offsets, uint64_t* offsets,
                                uint64_t* scan_state, hipStream_t stream);
+
+// Allocation-free contiguous prefix sum: at most kNoAllocCounters per-chunk
+// counters (fanout x chunks).  noalloc_scratch returns the current device's
+// copy of the library's static scratch array: counts[kNoAllocCounters] and
+// the scan state behind them (scan_state_words(kNoAllocCounters) words).
+constexpr uint64_t kNoAllocCounters = FIJ_NOALLOC_MAX_COUNTERS;
+hipError_t noalloc_scratch(uint32_t** counts, uint64_t** scan_state);
 
 // Histogram launch parameters (one relation, contiguous counts[F][chunks] or
 // chunked offsets[chunks][F]).

@@ -152,6 +152,7 @@ _proto("fij_partition_offsets_partition_len", C.c_int, C.POINTER(_PartitionOffse
 _proto("fij_partitioner_create", C.c_int, C.c_int, C.c_int, _RadixBits, _u32, _u32, _u64, C.POINTER(_vp))
 _proto("fij_partitioner_destroy", C.c_int, _vp)
 _proto("fij_prefix_sum", C.c_int, _vp, C.c_int, C.c_int, _vp, _u64, C.POINTER(_PartitionOffsets), _vp)
+_proto("fij_prefix_sum_noalloc", C.c_int, _vp, C.c_int, C.c_int, _vp, _u64, C.POINTER(_PartitionOffsets), _vp)
 _proto("fij_prefix_sum_and_copy_with_payload", C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _u64, _vp, _vp, _u64, C.POINTER(_PartitionOffsets), _vp)
 _proto("fij_prefix_sum_and_transform", C.c_int, _vp, C.c_int, C.c_int, _u32, C.POINTER(_PartitionedRelation), _vp, _vp, _u64, C.POINTER(_PartitionOffsets), _vp)
 _proto("fij_preallocate_partition_state", C.c_int, _vp, C.c_int, C.c_int)
@@ -570,6 +571,14 @@ class GpuRadixPartitioner:
         kt = _ktype(partition_attr.dtype)
         check(lib.fij_prefix_sum(self.h, int(radix_pass), int(kt), _ptr(partition_attr), partition_attr.numel(),
                                  C.byref(partition_offsets.s), _stream(stream)))
+
+    def prefix_sum_noalloc(self, radix_pass, partition_attr, partition_offsets, stream=None):
+        """prefix_sum without device allocation (fij_prefix_sum_noalloc): the
+        contiguous histogram's scratch is the library's static per-GPU array,
+        shared by every such call on that GPU."""
+        kt = _ktype(partition_attr.dtype)
+        check(lib.fij_prefix_sum_noalloc(self.h, int(radix_pass), int(kt), _ptr(partition_attr),
+                                         partition_attr.numel(), C.byref(partition_offsets.s), _stream(stream)))
 
     def prefix_sum_and_copy_with_payload(self, radix_pass, src_key, src_pay, dst_key, dst_pay, partition_offsets,
                                          stream=None):

@@ -215,6 +215,22 @@ fij_status fij_partitioner_destroy(fij_partitioner *p);
 fij_status fij_prefix_sum(fij_partitioner *p, fij_radix_pass pass, fij_type type, const void *partition_attr,
                           uint64_t len, fij_partition_offsets *offsets, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fij_prefix_sum with the same arguments, checks and
+ * results, but it calls no hipMalloc/hipFree.  The Contiguous histogram's
+ * per-chunk counters and look-back state live in a static device array of
+ * the library (FIJ_NOALLOC_MAX_COUNTERS counters, about 4 MiB, one copy per
+ * GPU, held until the process ends) instead of the partitioner's buffers;
+ * fanout x grid above FIJ_NOALLOC_MAX_COUNTERS is rejected with
+ * FIJ_INVALID_ARGUMENT.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other threads) race.  The Chunked
+ * histogram needs no scratch and runs as in fij_prefix_sum. */
+#define FIJ_NOALLOC_MAX_COUNTERS (1u << 20)
+fij_status fij_prefix_sum_noalloc(fij_partitioner *p, fij_radix_pass pass, fij_type type,
+                                  const void *partition_attr, uint64_t len, fij_partition_offsets *offsets,
+                                  void *stream);
+
 /* GpuRadixPartitioner::prefix_sum_and_copy_with_payload
  * (gpu_radix_partition.rs:431-451, impl :698-824; radix_partition.cu:207-315). */
 fij_status fij_prefix_sum_and_copy_with_payload(fij_partitioner *p, fij_radix_pass pass, fij_type type,
