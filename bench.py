@@ -264,9 +264,9 @@ def batched_main(a):
     plain_step = a.plain_sample or 16
     t_alloc = 0.0
     n_alloc = 0
-    t_batch = t_reused = 0.0
+    t_batch = t_reused = t_cold = 0.0
     gpu_batch = 0.0
-    stats_acc = {"small_joins": 0, "partitioned_joins": 0, "prepare_ms": 0.0}
+    stats_acc = {"small_joins": 0, "partitioned_joins": 0, "prepare_ms": 0.0, "grow_ms": 0.0, "record_ms": 0.0}
     checked = 0
     for w in waves:
         # inputs of the wave in one arena
@@ -285,13 +285,20 @@ def batched_main(a):
             fij.gen_iota(sp, 0)
             joins.append((rk, rp, sk, sp))
         fij.synchronize()
-        batch.execute(joins[: min(8, len(joins))])  # warm the kernels
+        # the first submission of a wave grows the size classes' buffers
+        # (device allocation); allocation is not part of the reference's
+        # throughput, so the wave is timed on its second submission and the
+        # first one is reported beside it
+        t0 = time.perf_counter()
+        cold, _ = batch.execute(joins)
+        t_cold += time.perf_counter() - t0
         t0 = time.perf_counter()
         got, st = batch.execute(joins)
         t_batch += time.perf_counter() - t0
         gpu_batch += st["gpu_ms"] / 1000.0
-        for k in ("small_joins", "partitioned_joins", "prepare_ms"):
+        for k in stats_acc:
             stats_acc[k] += st[k]
+        assert cold == got, "first and second submission of a wave differ"
         for (r, s_), g in zip(lens, got):
             assert g[0] == s_ and g[2] == s_ * (s_ - 1) // 2, "batched join result"
         # the plain per-join launch, one join at a time, every join
@@ -327,7 +334,10 @@ def batched_main(a):
         "config": {"workload": f"batched joins: {a.joins} int32 joins, |R| log-uniform 2^10..2^22, "
                                "|S| = |R| x U{4..16}, count + sums per join", "joins": a.joins,
                    "tuples": total, "waves": len(waves), "lanes": a.lanes, "parallelism": "single GPU"},
-        "batched": {"seconds": t_batch, "gpu_seconds": gpu_batch, "tuples_per_s": value, **stats_acc},
+        "batched": {"seconds": t_batch, "gpu_seconds": gpu_batch, "tuples_per_s": value, **stats_acc,
+                    "first_submission_seconds": t_cold,
+                    "note": "each wave timed on its second submission; the first also grows the size classes' "
+                            "buffers (device allocation, excluded as in the reference's throughput)"},
         "algorithmic_bytes": b_alg,
         "pipeline_roofline_frac": b_alg / gpu_batch / (HBM_PEAK_GBS * 1e9),
         "wall_roofline_frac": b_alg / t_batch / (HBM_PEAK_GBS * 1e9),

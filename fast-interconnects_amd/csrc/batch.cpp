@@ -221,7 +221,8 @@ fij_status grow_arena(ClassArena& a, KeyType kt, uint32_t b1, uint32_t b2, uint6
 
 // Records one size class's passes over its joins (join indices `ids`).
 fij_status enqueue_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, const std::vector<uint32_t>& ids,
-                         const fij_join_desc* joins, hipStream_t st, uint32_t lds_total, std::vector<uint8_t>& host) {
+                         const fij_join_desc* joins, hipStream_t st, uint32_t lds_total, std::vector<uint8_t>& host,
+                         double* grow_ms) {
   const uint32_t b1 = ck.b1, b2 = ck.b2;
   const KeyType kt = b->kt;
   const fij_type type = static_cast<fij_type>(b->cfg.type);
@@ -254,7 +255,9 @@ fij_status enqueue_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, c
   if (static_cast<uint64_t>(J) * F1 * F2 >= 0xFFFFFFFFull || tiles[0] >= 0xFFFFFFFFull || tiles[1] >= 0xFFFFFFFFull)
     return batch_fail(FIJ_INTEGER_OVERFLOW, "size class %u+%u: too many joins for one submission", b1, b2);
   const uint64_t units = static_cast<uint64_t>(J) * F1 * F2 + probe_total / 65536 + J + 1;
+  const auto t_grow = std::chrono::steady_clock::now();
   fij_status s = grow_arena(a, kt, b1, b2, J, rel1, rel2, tiles, words, units, st);
+  *grow_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_grow).count();
   if (s != FIJ_OK) return s;
   // descriptors
   const size_t n_hist = 2ull * J * sizeof(HistParams), n_scan = 2ull * J * sizeof(BatchScanJob),
@@ -557,8 +560,9 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
   }
   // every class's descriptors and (grown) buffers
   std::vector<uint8_t> host;
+  double grow_ms = 0;
   for (auto& kv : classes) {
-    fij_status s = enqueue_class(b, b->arenas[kv.first], kv.first, kv.second, joins, st, b->lds_total, host);
+    fij_status s = enqueue_class(b, b->arenas[kv.first], kv.first, kv.second, joins, st, b->lds_total, host, &grow_ms);
     if (s != FIJ_OK) return s;
   }
   if (n_small) {
@@ -566,6 +570,7 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
     BATCH_HIP(hipStreamSynchronize(st));
   }
   // record the submission (or launch it directly)
+  const auto t_rec = std::chrono::steady_clock::now();
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const bool use_graph = b->cfg.use_graph != 0;
@@ -649,6 +654,8 @@ fij_status fij_join_batch_execute(fij_join_batch* b, uint32_t n, const fij_join_
     stats->partitioned_joins = n_large;
     stats->size_classes = static_cast<uint32_t>(classes.size());
     stats->graph = use_graph ? 1u : 0u;
+    stats->grow_ms = static_cast<float>(grow_ms);
+    stats->record_ms = std::chrono::duration<float, std::milli>(t1 - t_rec).count();
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);

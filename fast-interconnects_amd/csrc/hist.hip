@@ -19,10 +19,14 @@ constexpr uint32_t kHistLdsBudget = 48 * 1024;
 constexpr uint32_t kWaveTmpWords = 32;  // u64 scratch for block scans
 
 
+// 16 bytes of keys, only as aligned as one key: gfx950 loads 16 bytes from
+// any 4-byte aligned global address, so a column that starts anywhere
+// (batched joins' columns sit at arbitrary element offsets of one arena)
+// still takes the vector path.
 template <typename K>
 struct VecN {
   static constexpr uint32_t N = 16 / sizeof(K);
-  typedef K type __attribute__((ext_vector_type(16 / sizeof(K))));
+  typedef K type __attribute__((ext_vector_type(16 / sizeof(K)), aligned(sizeof(K))));
 };
 
 // Shared tail of both histogram kernels: sub-histogram reduction and output.
@@ -74,10 +78,12 @@ __device__ __forceinline__ void hist_body(const HistParams& p, uint32_t c, uint3
   uint32_t* my = hist + ((tid / kWave) % S) * F;
 
   using V = VecN<K>;
-  const bool aligned = (reinterpret_cast<uintptr_t>(keys) % 16) == 0 &&
-                       (!COPY || ((reinterpret_cast<uintptr_t>(static_cast<const K*>(p.pays) + begin) |
-                                   reinterpret_cast<uintptr_t>(static_cast<K*>(p.dst_keys) + begin) |
-                                   reinterpret_cast<uintptr_t>(static_cast<K*>(p.dst_pays) + begin)) % 16) == 0);
+  // the copying variant also stores 16 bytes at a time, and keeps its vector
+  // path for columns that are all 16-byte aligned
+  const bool aligned = !COPY || ((reinterpret_cast<uintptr_t>(keys) |
+                                  reinterpret_cast<uintptr_t>(static_cast<const K*>(p.pays) + begin) |
+                                  reinterpret_cast<uintptr_t>(static_cast<K*>(p.dst_keys) + begin) |
+                                  reinterpret_cast<uintptr_t>(static_cast<K*>(p.dst_pays) + begin)) % 16) == 0;
   uint64_t scalar_from = 0;
   if (aligned) {
     const uint64_t nv = len / V::N;

@@ -23,6 +23,7 @@
 // wave's registers are deferred, split into work units and joined from
 // block-shared (join_block_kernel) or wave LDS tables (join_general_kernel).
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 #include <vector>
 
@@ -1402,16 +1403,54 @@ size_t pipeline_sort2_lds(KeyType t, uint32_t F2) {
   return sort2_lds_bytes<int64_t, 8192>(F2);
 }
 
+template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL>
+hipError_t launch_register_join(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
+  const size_t lds = join_lds_bytes<K, CAP, WAVES>();
+  auto kern = join_kernel<K, CAP, WAVES, BPL, SPL, true, true>;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
+  return hipGetLastError();
+}
+
+// Tuples per lane of one side of the register join (its gathers per item):
+// every step issues that many gathers whether the item fills them or not,
+// so a side whose items are small gets fewer.  The need is twice the
+// average item side (joins of one size class differ in size by up to 2x)
+// plus four standard deviations; 10 per lane (640 tuples) at most, which
+// with two buffers per side keeps 2 waves per SIMD at i64.  An item beyond
+// its side's capacity is deferred like any other large item.
+static uint32_t register_join_lanes(uint64_t side_tuples, uint64_t items) {
+  if (items == 0) return 10;
+  const double avg = 2.0 * static_cast<double>(side_tuples) / static_cast<double>(items);
+  const double need = avg + 4.0 * std::sqrt(avg) + 16.0;
+  return need <= 128.0 ? 2u : need <= 256.0 ? 4u : 10u;
+}
+
+template <typename K, uint32_t CAP, uint32_t WAVES>
+hipError_t launch_register_join_sized(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
+  const uint64_t items = static_cast<uint64_t>(d.F1) * (d.bits2 ? d.F2 : 1u);
+  const uint32_t bpl = register_join_lanes(d.build.n, items), spl = register_join_lanes(d.probe.n, items);
+  if (bpl == 10 && spl == 10) return launch_register_join<K, CAP, WAVES, 10, 10>(d, grid, stream);
+  if (spl == 10) {
+    if (bpl == 2) return launch_register_join<K, CAP, WAVES, 2, 10>(d, grid, stream);
+    return launch_register_join<K, CAP, WAVES, 4, 10>(d, grid, stream);
+  }
+  if (bpl == 10) {
+    if (spl == 2) return launch_register_join<K, CAP, WAVES, 10, 2>(d, grid, stream);
+    return launch_register_join<K, CAP, WAVES, 10, 4>(d, grid, stream);
+  }
+  return launch_register_join<K, CAP, WAVES, 4, 4>(d, grid, stream);
+}
+
 template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t GCAP, uint32_t GWAVES>
 hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, hipStream_t stream) {
-  constexpr uint32_t PL = 10;  // tuples per lane and side: two buffers fit 2 waves per SIMD
   if (!deferred) {
-    const size_t lds = join_lds_bytes<K, CAP, WAVES>();
-    auto kern = join_kernel<K, CAP, WAVES, PL, PL, true, true>;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(lds));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, stream, d);
-    return hipGetLastError();
+    // large tables (single-pass plans): one shape
+    if constexpr (CAP > 1024)
+      return launch_register_join<K, CAP, WAVES, 10, 10>(d, grid, stream);
+    else
+      return launch_register_join_sized<K, CAP, WAVES>(d, grid, stream);
   }
   // block-shared tables: 64 KB, two blocks of 512 threads per CU
   constexpr uint32_t BCAP = 65536 / (2 * sizeof(K));

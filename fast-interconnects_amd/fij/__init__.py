@@ -828,7 +828,8 @@ class JoinBatchConfig(C.Structure):
 
 class JoinBatchStats(C.Structure):
     _fields_ = [("gpu_ms", C.c_float), ("prepare_ms", C.c_float), ("total_ms", C.c_float), ("small_joins", _u32),
-                ("partitioned_joins", _u32), ("size_classes", _u32), ("graph", _u32)]
+                ("partitioned_joins", _u32), ("size_classes", _u32), ("graph", _u32), ("grow_ms", C.c_float),
+                ("record_ms", C.c_float)]
 
 
 _proto("fij_join_autotune", C.c_int, C.c_int, _u64, _u64, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32))

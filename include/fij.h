@@ -404,6 +404,8 @@ typedef struct fij_join_batch_stats {
   uint32_t partitioned_joins;
   uint32_t size_classes;
   uint32_t graph;
+  float grow_ms;     /* host, part of prepare_ms: growing size-class buffers (device allocation) */
+  float record_ms;   /* host, part of prepare_ms: recording and instantiating the graph */
 } fij_join_batch_stats;
 
 typedef struct fij_join_batch fij_join_batch;

@@ -133,3 +133,24 @@ def test_plan_deferred_many_runs(dtype):
     sp = orc.gen_iota(n_s, 0, dtype)
     got, _ = run_plan(rk, rp, sk, sp, 1, 12)
     assert got == (n_s, int(sk.astype(np.int64).sum()), n_s * (n_s - 1) // 2)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("log2r,log2s", [(16, 20), (17, 20), (20, 16), (20, 17), (16, 16), (20, 20)])
+def test_plan_register_join_shapes(dtype, log2r, log2s):
+    """The register join gathers per side what the items need (2, 4 or 10
+    tuples per lane from the average item side): build-light, probe-light,
+    both light and both full shapes at 6 + 5 bits, each against the
+    sort-merge join; the Zipf probe side puts heavy items past a light
+    side's capacity, which then go to the deferred kernels."""
+    n_r, n_s = 1 << log2r, 1 << log2s
+    rk = orc.gen_perm(n_r, 5, dtype, 1)
+    rp = orc.gen_uniform(n_r, 5, 2, -(1 << 20), 1 << 21, dtype)
+    for probe in ("uniform", "zipf"):
+        if probe == "uniform":
+            sk = orc.gen_uniform(n_s, 6, 1, 1, n_r + n_r // 4, dtype)  # a fifth of the probes miss
+        else:
+            sk = orc.gen_zipf_keys(n_s, n_r, 1.0, 6, 1, 5, dtype=dtype)  # ranks over R's keys
+        sp = orc.gen_iota(n_s, 3, dtype)
+        got, _ = run_plan(rk, rp, sk, sp, 6, 5)
+        assert got == orc.sort_merge_join(rk, rp, sk, sp), probe
