@@ -342,7 +342,10 @@ hipError_t launch_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, hi
   const uint64_t tiles = static_cast<uint64_t>(a.sub_tiles[0]) + a.sub_tiles[1];
   const uint32_t sort_grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(b->cus, tiles)));
   if ((e = launch_pipeline_sort2(a.d, kt, sort_grid, st)) != hipSuccess) return e;
-  if ((e = launch_pipeline_join(a.d, kt, false, false, 2 * b->cus, st)) != hipSuccess) return e;
+  // the register join: four blocks per CU, as many as the lighter join shapes
+  // of batched classes (fewer registers, smaller tables) keep resident; the
+  // others start as earlier blocks finish and find the items taken
+  if ((e = launch_pipeline_join(a.d, kt, false, false, 4 * b->cus, st)) != hipSuccess) return e;
   return launch_pipeline_join(a.d, kt, false, true, 2 * b->cus, st);
 }
 

@@ -1427,20 +1427,24 @@ static uint32_t register_join_lanes(uint64_t side_tuples, uint64_t items) {
   return need <= 128.0 ? 2u : need <= 256.0 ? 4u : 10u;
 }
 
+// The wave table holds a whole build side of the register join (at most
+// BPL * 64 tuples) at a load factor of at most 3/4: 256 slots for 2 tuples
+// per lane, 512 for 4, CAP (1024) for 10.  A smaller table leaves LDS for
+// more waves per CU.
 template <typename K, uint32_t CAP, uint32_t WAVES>
 hipError_t launch_register_join_sized(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
   const uint64_t items = static_cast<uint64_t>(d.F1) * (d.bits2 ? d.F2 : 1u);
   const uint32_t bpl = register_join_lanes(d.build.n, items), spl = register_join_lanes(d.probe.n, items);
   if (bpl == 10 && spl == 10) return launch_register_join<K, CAP, WAVES, 10, 10>(d, grid, stream);
   if (spl == 10) {
-    if (bpl == 2) return launch_register_join<K, CAP, WAVES, 2, 10>(d, grid, stream);
-    return launch_register_join<K, CAP, WAVES, 4, 10>(d, grid, stream);
+    if (bpl == 2) return launch_register_join<K, CAP / 4, WAVES, 2, 10>(d, grid, stream);
+    return launch_register_join<K, CAP / 2, WAVES, 4, 10>(d, grid, stream);
   }
   if (bpl == 10) {
     if (spl == 2) return launch_register_join<K, CAP, WAVES, 10, 2>(d, grid, stream);
     return launch_register_join<K, CAP, WAVES, 10, 4>(d, grid, stream);
   }
-  return launch_register_join<K, CAP, WAVES, 4, 4>(d, grid, stream);
+  return launch_register_join<K, CAP / 2, WAVES, 4, 4>(d, grid, stream);
 }
 
 template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t GCAP, uint32_t GWAVES>
