@@ -81,8 +81,13 @@ void partition_t(const K* keys, const K* pays, uint64_t n, uint32_t bits, uint32
       run += cnt;
     }
   }
-  // lines are 64-byte aligned in memory only when the output is
-  const bool lines = reinterpret_cast<uintptr_t>(out) % 64 == 0;
+  // A buffer stands for one 64-byte line of memory: tuple i of the output
+  // lies in line (i + sh) / L, sh = the output's offset into its first line
+  // in tuples (any output aligned to its tuple size; otherwise every line is
+  // copied).
+  const uintptr_t addr = reinterpret_cast<uintptr_t>(out);
+  const bool lines = addr % sizeof(T) == 0;
+  const uint32_t sh = lines ? static_cast<uint32_t>(addr % 64 / sizeof(T)) : 0u;
   parallel_for(threads, [&](uint32_t t) {
     const uint64_t b = std::min(n, per * t), e = std::min(n, b + per);
     uint64_t* cur = &hist[static_cast<size_t>(t) * F];  // first tuple of the buffered line
@@ -90,8 +95,8 @@ void partition_t(const K* keys, const K* pays, uint64_t n, uint32_t bits, uint32
     T* buf = reinterpret_cast<T*>(storage.data() + (-reinterpret_cast<uintptr_t>(storage.data()) & 63));
     std::vector<uint32_t> fill(F), lo(F);  // buffered slots [lo, fill) of the line
     for (uint32_t d = 0; d < F; ++d) {
-      lo[d] = fill[d] = lines ? static_cast<uint32_t>(cur[d] % L) : 0u;
-      cur[d] -= lo[d];
+      lo[d] = fill[d] = lines ? static_cast<uint32_t>((cur[d] + sh) % L) : 0u;
+      cur[d] -= lo[d];  // out + cur[d] is 64-byte aligned when lines
     }
     for (uint64_t i = b; i < e; ++i) {
       const uint32_t d = host_digit(keys[i], mask, ignore);

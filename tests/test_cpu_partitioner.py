@@ -90,10 +90,11 @@ def test_partition_threads_keep_each_partitions_multiset(threads):
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
 @pytest.mark.parametrize("threads", [1, 5])
 def test_partition_line_aligned_and_misaligned_output(dtype, threads):
-    """With a 64-byte aligned output every whole line of a thread's range is
-    streamed (non-temporal stores) and only the lines it shares with another
-    thread or with padding are copied; with an output one tuple off the line
-    grid every line is copied.  Both give the same relation."""
+    """Every whole 64-byte line of a thread's range is streamed
+    (non-temporal stores) and only the lines it shares with another thread or
+    with padding are copied, whether the output starts on a line or one tuple
+    into it; an output not aligned to its tuple size (half a tuple off) copies
+    every line.  All give the same relation."""
     n = 200_003
     keys = keys_of(n, dtype, seed=12, hi=1 << 30)
     pays = orc.gen_iota(n, 0, dtype)
@@ -101,15 +102,16 @@ def test_partition_line_aligned_and_misaligned_output(dtype, threads):
     rows = n + (1 << bits) * orc.pad_len(dtype)
     tb = 2 * np.dtype(dtype).itemsize
     got = []
-    for shift in (0, tb):
+    for shift in (0, tb, tb // 2):
         raw = np.zeros(rows * tb + 128, np.uint8)
         base = (-raw.ctypes.data) % 64 + shift
         out = raw[base: base + rows * tb].view(dtype).reshape(rows, 2)
         assert (out.ctypes.data % 64 == 0) == (shift == 0)
         k, v, off = fij.cpu_radix_partition(keys, pays, bits, threads=threads, out=out)
         got.append((k.copy(), v.copy(), off))
-    for a, b in zip(got[0], got[1]):
-        np.testing.assert_array_equal(a, b)
+    for other in got[1:]:
+        for a, b in zip(got[0], other):
+            np.testing.assert_array_equal(a, b)
     ref_k, ref_v, ref_off = fij.cpu_radix_partition(keys, pays, bits, threads=threads)
     np.testing.assert_array_equal(got[0][2], ref_off)
     np.testing.assert_array_equal(got[0][0], ref_k)
