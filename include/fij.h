@@ -376,9 +376,11 @@ fij_status fij_join_plan_execute(fij_join_plan *plan, const void *build_keys, co
  * in two passes with the radix bits of its size class (fij_join_autotune),
  * all joins of a class together: one launch per pass (histograms, offsets,
  * scatter, pass 2, join) covers every join of the class.  The submission is
- * recorded into one HIP graph when use_graph is set.  Results are per join,
- * identical to joining each pair alone.  Columns need only be aligned to
- * their element size. */
+ * recorded into one HIP graph when use_graph is set.  The size classes are
+ * independent and are spread over `lanes` streams (forked from the
+ * submission stream and joined back into it), so classes that do not fill
+ * the GPU overlap.  Results are per join, identical to joining each pair
+ * alone.  Columns need only be aligned to their element size. */
 
 typedef struct fij_join_desc {
   const void *build_keys;
