@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=2000, help="served: requests in the Poisson stream")
     ap.add_argument("--load", type=float, default=0.8, help="served: offered load as a fraction of capacity")
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="served: HBM budget (GB)")
+    ap.add_argument("--table-build-max", type=int, default=1 << 18,
+                    help="served: builds of at most this many tuples kept as resident hash tables (0 = none)")
     ap.add_argument("--cache-gb", type=float, default=2.0, help="ooc: device cache for pass-1 partitions (GB)")
     ap.add_argument("--pass1-cpu", action="store_true", help="ooc: pass 1 on host threads")
     ap.add_argument("--probe-slices", type=int, default=0,
@@ -355,7 +357,9 @@ def batched_main(a):
 def served_main(a):
     """Served joins: a Poisson stream of requests against a pool of a.builds
     int64 build sides (|R| log-uniform 2^12..2^26, keys a permutation of
-    1..|R|, payload = key), partitioned once and kept resident.  Each request
+    1..|R|, payload = key), kept resident: builds of at most
+    a.table_build_max tuples as open-addressing hash tables, larger ones
+    partitioned once at registration.  Each request
     brings one of a.probe_inputs probe sides (|S| log-uniform 2^14..2^28,
     keys uniform over 1..1.25|R|: 80% join, 20% miss; payload = row index)
     held in device memory.  Slots and resident builds are charged against
@@ -374,7 +378,8 @@ def served_main(a):
     n_p = np.floor(2.0 ** rng.uniform(14, 28, a.probe_inputs)).astype(np.int64)
     which = rng.integers(0, a.builds, a.probe_inputs)
     slot_classes = [(8, 1 << 20), (4, 1 << 24), (2, 1 << 28)]
-    srv = fij.JoinServer(np.int64, slot_classes, max_probe_len=1 << 28, hbm_budget=int(a.hbm_gb * 1e9))
+    srv = fij.JoinServer(np.int64, slot_classes, max_probe_len=1 << 28, hbm_budget=int(a.hbm_gb * 1e9),
+                         table_build_max=a.table_build_max)
     scratch_k = fij.DeviceArray(int(n_b.max()), np.int64)
     scratch_p = fij.DeviceArray(int(n_b.max()), np.int64)
     t0 = time.perf_counter()
@@ -491,7 +496,8 @@ def served_main(a):
                                f"{a.builds} resident int64 builds |R| log-uniform 2^12..2^26, probes |S| "
                                "log-uniform 2^14..2^28 (80% matching keys), count + sums per request",
                    "builds": a.builds, "probe_inputs": a.probe_inputs, "requests": len(stream),
-                   "slot_classes": slot_classes, "hbm_budget_gb": a.hbm_gb, "parallelism": "single GPU"},
+                   "slot_classes": slot_classes, "hbm_budget_gb": a.hbm_gb,
+                   "table_build_max": a.table_build_max, "parallelism": "single GPU"},
         "tuples_note": "value counts |R| + |S| per request (the joined relations); the resident |R| is "
                        "partitioned once at registration",
         "probe_tuples_per_s": float(n_p[stream].sum()) / wall,

@@ -208,6 +208,11 @@ hipError_t launch_np_probe(KeyType t, JoinScheme s, const void* table, uint64_t 
                            JoinAggregate* aggregates, uint32_t grid, uint32_t block,
                            hipStream_t stream);
 
+// Probe of a resident linear-probing table of 2^lg entries; block b writes its
+// aggregate to out[b] (grid blocks of 256 threads).
+hipError_t launch_np_probe_resident(KeyType t, const void* table, uint32_t lg, const void* keys, const void* pays,
+                                    uint64_t n, JoinAggregate* out, uint32_t grid, hipStream_t stream);
+
 // ---- data generation (datagen.hip) -----------------------------------------
 hipError_t launch_gen_perm_keys(KeyType t, void* out, uint64_t n, uint64_t seed, int64_t base,
                                 hipStream_t stream);

@@ -138,6 +138,74 @@ __global__ __launch_bounds__(1024) void np_probe_kernel(const Tuple<K>* table, u
   if (aggregates) np_block_aggregate(aggregates, cnt, sb, sp);
 }
 
+// Probe of a resident linear-probing table of 2^lg entries (a served build
+// side that fits the GPU's cache): U probes in flight per thread, their first
+// slots loaded together; every match counts, probing ends at a free slot.
+// Block b writes (not adds) its {count, SUM(build.value), SUM(probe.value)}
+// to out[b].
+template <typename K, uint32_t U>
+__global__ __launch_bounds__(256) void np_probe_resident_kernel(const Tuple<K>* table, uint32_t lg, const K* keys,
+                                                                const K* pays, uint64_t n, JoinAggregate* out) {
+  const uint64_t mask = (1ull << lg) - 1;
+  const uint64_t T = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  uint64_t cnt = 0, sb = 0, sp = 0;
+  for (uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i0 < n; i0 += U * T) {
+    K key[U], pay[U];
+    uint64_t h[U];
+    Tuple<K> e[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint64_t i = i0 + u * T;
+      key[u] = i < n ? keys[i] : null_key<K>();
+      pay[u] = i < n ? pays[i] : K(0);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      h[u] = mult_shift_hash<K>(key[u], lg);
+      e[u] = load_tuple<K>(table + h[u]);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if (key[u] == null_key<K>()) continue;  // past the end, or a null key: never joins
+      uint64_t hh = h[u];
+      Tuple<K> x = e[u];
+      for (uint64_t a = 0; a <= mask; ++a) {
+        if (x.key == key[u]) {
+          cnt += 1;
+          sb += static_cast<uint64_t>(static_cast<int64_t>(x.value));
+          sp += static_cast<uint64_t>(static_cast<int64_t>(pay[u]));
+        } else if (x.key == null_key<K>()) {
+          break;
+        }
+        hh = (hh + 1) & mask;
+        x = load_tuple<K>(table + hh);
+      }
+    }
+  }
+  __shared__ uint64_t red[3][256 / kWave];
+  const uint32_t w = threadIdx.x / kWave;
+  const uint64_t wc = wave_broadcast_last(wave_inclusive_scan(cnt));
+  const uint64_t wb = wave_broadcast_last(wave_inclusive_scan(sb));
+  const uint64_t wp = wave_broadcast_last(wave_inclusive_scan(sp));
+  if (lane_id() == 0) {
+    red[0][w] = wc;
+    red[1][w] = wb;
+    red[2][w] = wp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t a = 0, b = 0, d = 0;
+    for (uint32_t i = 0; i < blockDim.x / kWave; ++i) {
+      a += red[0][i];
+      b += red[1][i];
+      d += red[2][i];
+    }
+    out[blockIdx.x].count = a;
+    out[blockIdx.x].sum_build = b;
+    out[blockIdx.x].sum_probe = d;
+  }
+}
+
 template <typename K>
 hipError_t build_t(JoinScheme s, bool selective, void* table, uint64_t entries, const void* keys, const void* pays, uint64_t n,
                    uint32_t grid, uint32_t block, hipStream_t stream) {
@@ -189,6 +257,20 @@ hipError_t launch_np_build(KeyType t, JoinScheme s, bool selective, void* table,
                            const void* pays, uint64_t n, uint32_t grid, uint32_t block, hipStream_t stream) {
   if (t == KeyType::I32) return build_t<int32_t>(s, selective, table, entries, keys, pays, n, grid, block, stream);
   return build_t<int64_t>(s, selective, table, entries, keys, pays, n, grid, block, stream);
+}
+
+hipError_t launch_np_probe_resident(KeyType t, const void* table, uint32_t lg, const void* keys, const void* pays,
+                                    uint64_t n, JoinAggregate* out, uint32_t grid, hipStream_t stream) {
+  constexpr uint32_t U = 4;
+  if (t == KeyType::I32)
+    hipLaunchKernelGGL((np_probe_resident_kernel<int32_t, U>), dim3(grid), dim3(256), 0, stream,
+                       static_cast<const Tuple<int32_t>*>(table), lg, static_cast<const int32_t*>(keys),
+                       static_cast<const int32_t*>(pays), n, out);
+  else
+    hipLaunchKernelGGL((np_probe_resident_kernel<int64_t, U>), dim3(grid), dim3(256), 0, stream,
+                       static_cast<const Tuple<int64_t>*>(table), lg, static_cast<const int64_t*>(keys),
+                       static_cast<const int64_t*>(pays), n, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_np_probe(KeyType t, JoinScheme s, const void* table, uint64_t entries, const void* keys,

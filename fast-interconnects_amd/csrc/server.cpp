@@ -86,6 +86,10 @@ struct ResidentBuild {
   uint32_t bits1 = 0, bits2 = 0;
   uint64_t n = 0;
   PipelineSide side{};
+  // a build side of at most table_build_max tuples: one linear-probing table
+  // of 2^table_lg entries instead of partitions (null: partitioned)
+  void* table = nullptr;
+  uint32_t table_lg = 0;
   std::vector<void*> allocs;
   uint64_t bytes = 0;
 };
@@ -173,6 +177,16 @@ uint32_t chunks_for(KeyType kt, uint64_t n, uint32_t cap) {
 // Launches request r on slot sl (scheduler thread, server lock held).
 fij_status launch_request(fij_join_server* s, Slot& sl, const Request& r) {
   const ResidentBuild& b = *s->builds[r.build];
+  if (b.table) {
+    // a resident table: one probe launch, no partitioning of the probe side
+    hipStream_t st = sl.stream;
+    SRV_HIP(launch_np_probe_resident(s->kt, b.table, b.table_lg, r.keys, r.pays, r.len, sl.aggregates, s->join_grid,
+                                     st));
+    SRV_HIP(launch_sum_aggregates(sl.aggregates, s->join_grid, sl.d_result, st));
+    SRV_HIP(hipMemcpyAsync(sl.h_result, sl.d_result, sizeof(JoinAggregate), hipMemcpyDeviceToHost, st));
+    SRV_HIP(hipEventRecord(sl.done, st));
+    return FIJ_OK;
+  }
   PipelineDevice d = base_device(s, b);
   d.probe = sl.probe;
   d.probe.keys = r.keys;
@@ -374,6 +388,29 @@ fij_status fij_join_server_destroy(fij_join_server* s) {
   return FIJ_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Charges a built side against the HBM budget and makes it resident.
+fij_status register_build(fij_join_server* s, std::unique_ptr<ResidentBuild> b, uint32_t* build_id) {
+  b->bytes = allocated_bytes(b->allocs);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->cfg.hbm_budget && s->budget_used + b->bytes > s->cfg.hbm_budget) {
+    free_all(b->allocs);
+    return server_fail(FIJ_OUT_OF_MEMORY, "resident build of %llu bytes exceeds the HBM budget",
+                       static_cast<unsigned long long>(b->bytes));
+  }
+  s->budget_used += b->bytes;
+  *build_id = static_cast<uint32_t>(s->builds.size());
+  s->builds.push_back(std::move(b));
+  return FIJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 fij_status fij_join_server_add_build(fij_join_server* s, const void* keys, const void* payloads, uint64_t len,
                                      uint32_t* build_id) {
   if (!s || !build_id || (len && (!keys || !payloads))) return server_fail(FIJ_INVALID_ARGUMENT, "invalid argument");
@@ -388,6 +425,30 @@ fij_status fij_join_server_add_build(fij_join_server* s, const void* keys, const
     *build_id = static_cast<uint32_t>(s->builds.size());
     s->builds.push_back(std::move(b));
     return FIJ_OK;
+  }
+  if (s->cfg.table_build_max && len <= s->cfg.table_build_max) {
+    // Small enough for the GPU's cache: one linear-probing table of at least
+    // twice its tuples (the no-partitioning join's build,
+    // no_partitioning_join.cu:39-132); a request then probes it in one
+    // launch, its probe side read once instead of partitioned twice.
+    uint32_t lg = 1;
+    while ((1ull << lg) < 2 * len) ++lg;
+    const uint64_t entries = 1ull << lg;
+    void* tab = nullptr;
+    hipError_t e = hipMalloc(&tab, entries * tuple_bytes(s->kt));
+    if (e != hipSuccess) return hip_status(e, "hipMalloc");
+    b->allocs.push_back(tab);
+    if ((e = launch_np_table_init(s->kt, tab, entries, true, s->setup)) != hipSuccess ||
+        (e = launch_np_build(s->kt, JoinScheme::LinearProbing, false, tab, entries, keys, payloads, len, 4 * s->cus,
+                             256, s->setup)) != hipSuccess ||
+        (e = hipStreamSynchronize(s->setup)) != hipSuccess) {
+      free_all(b->allocs);
+      return hip_status(e, "building the resident table");
+    }
+    b->table = tab;
+    b->table_lg = lg;
+    b->n = len;
+    return register_build(s, std::move(b), build_id);
   }
   // A resident build side serves probes of any length up to max_probe_len:
   // its bits come from its own length (pairs of about 512 build tuples), with
@@ -450,17 +511,7 @@ fij_status fij_join_server_add_build(fij_join_server* s, const void* keys, const
   b->side.scan1 = nullptr;
   b->side.keys = nullptr;
   b->side.pays = nullptr;
-  b->bytes = allocated_bytes(b->allocs);
-  std::lock_guard<std::mutex> lk(s->mu);
-  if (s->cfg.hbm_budget && s->budget_used + b->bytes > s->cfg.hbm_budget) {
-    free_all(b->allocs);
-    return server_fail(FIJ_OUT_OF_MEMORY, "resident build of %llu bytes exceeds the HBM budget",
-                       static_cast<unsigned long long>(b->bytes));
-  }
-  s->budget_used += b->bytes;
-  *build_id = static_cast<uint32_t>(s->builds.size());
-  s->builds.push_back(std::move(b));
-  return FIJ_OK;
+  return register_build(s, std::move(b), build_id);
 }
 
 fij_status fij_join_server_submit(fij_join_server* s, uint32_t build_id, const void* probe_keys,

@@ -921,7 +921,8 @@ SERVER_MAX_SLOT_CLASSES = 4
 
 class JoinServerConfig(C.Structure):
     _fields_ = [("type", _u32), ("slot_classes", _u32), ("slots", _u32 * SERVER_MAX_SLOT_CLASSES),
-                ("slot_probe_len", _u64 * SERVER_MAX_SLOT_CLASSES), ("max_probe_len", _u64), ("hbm_budget", _u64)]
+                ("slot_probe_len", _u64 * SERVER_MAX_SLOT_CLASSES), ("max_probe_len", _u64), ("hbm_budget", _u64),
+                ("table_build_max", _u64)]
 
 
 class JoinRequestTimes(C.Structure):
@@ -943,7 +944,7 @@ class JoinServer:
     wait() returns) and returns a request id; wait() returns
     ((count, sum_build, sum_probe), {"queued_ms", "latency_ms"})."""
 
-    def __init__(self, dtype, slot_classes, max_probe_len, hbm_budget=0):
+    def __init__(self, dtype, slot_classes, max_probe_len, hbm_budget=0, table_build_max=0):
         if not 0 < len(slot_classes) <= SERVER_MAX_SLOT_CLASSES:
             raise ValueError("between 1 and %d slot classes" % SERVER_MAX_SLOT_CLASSES)
         cfg = JoinServerConfig()
@@ -954,6 +955,7 @@ class JoinServer:
             cfg.slot_probe_len[i] = cap
         cfg.max_probe_len = max_probe_len
         cfg.hbm_budget = hbm_budget
+        cfg.table_build_max = table_build_max  # smaller build sides: one resident hash table
         h = _vp()
         check(lib.fij_join_server_create(C.byref(cfg), C.byref(h)))
         self.h = h.value

@@ -473,6 +473,10 @@ typedef struct fij_join_server_config {
   uint64_t slot_probe_len[FIJ_SERVER_MAX_SLOT_CLASSES];  /* largest probe side of a class's slot */
   uint64_t max_probe_len;                                /* largest probe side expected (radix bits of builds) */
   uint64_t hbm_budget;                                   /* bytes; 0 = unlimited */
+  uint64_t table_build_max;                              /* build sides of at most this many tuples are kept as
+                                                            one linear-probing table (2x their tuples, a power of
+                                                            two) and probed in one launch; 0 = every build side
+                                                            is partitioned */
 } fij_join_server_config;
 
 typedef struct fij_join_request_times {

@@ -31,11 +31,15 @@ def make_probe(j, n_b, n, dtype):
     return sk, sp
 
 
+@pytest.mark.parametrize("table_build_max", [0, 1 << 17])
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
-def test_server_alone_equals_mix_equals_oracle(dtype):
+def test_server_alone_equals_mix_equals_oracle(dtype, table_build_max):
+    """table_build_max = 2^17: the two smaller builds (one with duplicates
+    and null keys) are resident hash tables, the larger one partitioned, in
+    the same mix."""
     builds = [make_build(0, 4096, dtype), make_build(1, 1 << 17, dtype, dup=True), make_build(2, 300000, dtype),
               make_build(3, 0, dtype)]
-    srv = fij.JoinServer(dtype, [(3, 1 << 16), (2, 1 << 20)], max_probe_len=1 << 20)
+    srv = fij.JoinServer(dtype, [(3, 1 << 16), (2, 1 << 20)], max_probe_len=1 << 20, table_build_max=table_build_max)
     dev_builds = [tuple(to_dev(a) if len(a) else fij.DeviceArray(1, dtype).view(0, 0) for a in b) for b in builds]
     ids = [srv.add_build(*b) for b in dev_builds]
     del dev_builds  # resident: the input columns are no longer needed
@@ -66,10 +70,11 @@ def test_server_alone_equals_mix_equals_oracle(dtype):
     assert fij.device_status() == 0
 
 
-def test_server_concurrent_submitters():
+@pytest.mark.parametrize("table_build_max", [0, 1 << 16])
+def test_server_concurrent_submitters(table_build_max):
     dtype = np.int64
     build = make_build(5, 50000, dtype)
-    srv = fij.JoinServer(dtype, [(4, 1 << 18)], max_probe_len=1 << 18)
+    srv = fij.JoinServer(dtype, [(4, 1 << 18)], max_probe_len=1 << 18, table_build_max=table_build_max)
     bid = srv.add_build(*(to_dev(a) for a in build))
     probes = [make_probe(40 + j, 50000, int(2 ** (12 + j % 6)), dtype) for j in range(16)]
     devs = [tuple(to_dev(a) for a in p) for p in probes]
