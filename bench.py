@@ -171,6 +171,47 @@ def stream_copy_gbs(fij, nbytes, reps=5):
     return 2 * nbytes / (ms / 1000.0) / 1e9
 
 
+def access_pattern_ceilings(fij, n, bits1, bits2, chunks, scatter_ms, join_ms, reps=5):
+    """Times of the scatter's and the join's memory patterns alone (the
+    library's calibration kernels, HIP events, median of `reps`), beside the
+    kernels' own times: how close each kernel is to what its access pattern
+    costs on this GPU.  The line-scatter copy moves the scatter's bytes (n
+    16-byte words read, n written as whole 128-byte lines to 2^bits1
+    partitions x `chunks` chunk pieces); the run gathers read 2n words as the
+    join's items (2^(bits1+bits2) per relation) in runs of one pass-2 tile
+    each, 128-byte aligned or shifted by one 16-byte word."""
+    def timed(fn):
+        e0, e1 = fij.Event(), fij.Event()
+        ms = []
+        fn()
+        for _ in range(reps):
+            e0.record()
+            fn()
+            e1.record()
+            ms.append(e1.elapsed_ms(e0))
+        return float(np.median(ms))
+
+    src = fij.DeviceArray(4 * n, np.int64)  # 2n 16-byte words
+    dst = fij.DeviceArray(2 * n, np.int64)
+    src.fill(3)
+    f1, f2 = 1 << bits1, 1 << bits2
+    line_ms = timed(lambda: fij.calib_line_scatter(dst, src, f1, chunks, nbytes=16 * n))
+    del dst
+    ck = fij.DeviceArray.zeros(1, np.uint32)
+    part = n // f1
+    run = 8192 // f2  # the plan's pass-2 tiles hold 8192 tuples: one run per digit and tile
+    gather = {}
+    for name, r, shift in ((f"runs{run}_aligned", run, 0), (f"runs{run}_shifted", run, 1),
+                           (f"runs{2 * run}_shifted", 2 * run, 1)):
+        gather[name] = timed(lambda: fij.calib_run_gather(src, part, f2, r, shift, ck))
+    return {
+        "line_scatter_copy_ms": line_ms,
+        "scatter_over_line_scatter_copy": scatter_ms / line_ms,
+        "run_gather_ms": gather,
+        "join_over_shifted_run_gather": join_ms / gather[f"runs{run}_shifted"],
+    }
+
+
 def skew_main(a):
     """Skewed join: R = 2^27 int32 tuples with unique permuted keys 1..2^27,
     S = 2^skew_log2s int32 tuples with Zipf(--zipf) keys over R's keys
@@ -865,8 +906,11 @@ def main():
         "result": {"count": res[0], "sum_build": res[1], "sum_probe": res[2]},
     }
     out["result_check"] = "count, SUM(R.value) and SUM(S.value) equal their closed forms"
+    del rk, rp, sk, sp, plan
+    out["pattern_ceilings"] = access_pattern_ceilings(
+        fij, n, a.bits1, a.bits2, a.chunks or fij.device_limits()["compute_units"],
+        kernels["scatter_swwc_kernel"]["ms"], kernels["join_kernel"]["ms"])
     if not a.no_cpu:
-        del rk, rp, sk, sp, plan
         threads = a.cpu_threads or min(HOST_CPU_SHARE, host_cpu()[0])
         cpu_log2n = a.cpu_log2n or a.log2n
         out["cpu_baseline"], cpu_res = cpu_baseline(cpu_log2n, a.bits1, a.bits2, threads)

@@ -159,6 +159,83 @@ __global__ __launch_bounds__(256) void copy_kernel(const u32x4* __restrict__ src
     if (base + k * 256 < n) __builtin_nontemporal_store(v[k], dst + base + k * 256);
 }
 
+// The pass-1 scatter's write pattern without its placement work: block c
+// reads chunk c (words / chunks consecutive 16-byte words, 1024 per step,
+// eight steps of loads in flight) and writes every step's 128 lines of 128
+// bytes round-robin to `partitions` regions, each holding the chunks' pieces
+// in chunk order -- the layout and line order of a uniform pass-1 partition.
+// The host checks that every region and piece divides evenly.
+constexpr uint32_t kCalibThreads = 1024, kCalibDepth = 8;
+__global__ __launch_bounds__(kCalibThreads) void line_scatter_kernel(const u32x4* __restrict__ src,
+                                                                      u32x4* __restrict__ dst, uint64_t words,
+                                                                      uint32_t partitions) {
+  const uint64_t chunk = words / gridDim.x;  // words per block
+  const uint64_t piece = chunk / partitions;  // words per (partition, chunk)
+  const uint64_t region = piece * gridDim.x;
+  const uint32_t steps = static_cast<uint32_t>(chunk / kCalibThreads);
+  const uint32_t t = threadIdx.x, line = t / 8, word = t % 8;
+  const u32x4* in = src + blockIdx.x * chunk;
+  for (uint32_t s0 = 0; s0 < steps; s0 += kCalibDepth) {
+    u32x4 v[kCalibDepth];
+#pragma unroll
+    for (uint32_t k = 0; k < kCalibDepth; ++k)
+      v[k] = __builtin_nontemporal_load(in + static_cast<uint64_t>(s0 + k) * kCalibThreads + t);
+#pragma unroll
+    for (uint32_t k = 0; k < kCalibDepth; ++k) {
+      // the block's line number l goes to partition l mod P, its (l / P)-th line there
+      const uint64_t l = static_cast<uint64_t>(s0 + k) * (kCalibThreads / 8) + line;
+      const uint64_t p = l % partitions, m = l / partitions;
+      __builtin_nontemporal_store(v[k], dst + p * region + blockIdx.x * piece + m * 8 + word);
+    }
+  }
+}
+
+// The register join's gather shape without its table work: one wave per
+// item (p, q) of `items_per_part` = part_words / f2 words, read as runs of
+// run_words words, run r of the item at tile r of partition p (a tile is f2
+// runs), every run but a tile's last shifted by `shift` words -- runs that
+// start inside a cache line, as real pass-2 runs do.  Each lane loads
+// ceil(item / 64) words per item (lanes past the end repeat the last one),
+// two items in flight; a per-wave xor of the loaded words keeps the loads.
+__global__ __launch_bounds__(256) void run_gather_kernel(const u32x4* __restrict__ src, uint64_t words,
+                                                         uint32_t part_words, uint32_t f2, uint32_t run_words,
+                                                         uint32_t shift, uint32_t* __restrict__ out) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t item_words = part_words / f2;
+  const uint64_t items = words / part_words * f2;
+  const uint32_t loads = (item_words + 63) / 64;
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * 4;
+  uint32_t acc = 0;
+  auto word = [&](uint64_t i, uint32_t k) {
+    const uint64_t p = i / f2;
+    const uint32_t q = static_cast<uint32_t>(i % f2);
+    const uint32_t sh = q + 1 < f2 ? shift : 0u;
+    const uint32_t e = min(k * 64 + lane, item_words - 1);
+    return p * part_words + static_cast<uint64_t>(e / run_words) * (f2 * run_words) +
+           static_cast<uint64_t>(q) * run_words + e % run_words + sh;
+  };
+  // two items per round, all their loads issued before either is used
+  for (uint64_t i = blockIdx.x * 4ull + threadIdx.x / 64; i < items; i += 2 * waves) {
+    const bool two = i + waves < items;
+    u32x4 v[16], u[16];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k)
+      if (k < loads) v[k] = src[word(i, k)];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k)
+      if (k < loads && two) u[k] = src[word(i + waves, k)];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k)
+      if (k < loads) acc ^= v[k].x ^ v[k].y ^ v[k].z ^ v[k].w;
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k)
+      if (k < loads && two) acc ^= u[k].x ^ u[k].y ^ u[k].z ^ u[k].w;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc ^= static_cast<uint32_t>(__shfl_xor(static_cast<int>(acc), off));
+  if (lane == 0) atomicXor(out, acc);
+}
+
 uint32_t grid_for(uint64_t n) {
   uint64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -219,6 +296,20 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t words16, hipStream_t
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(copy_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream,
                      static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), words16);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_scatter(void* dst, const void* src, uint64_t words16, uint32_t partitions, uint32_t chunks,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(line_scatter_kernel, dim3(chunks), dim3(kCalibThreads), 0, stream,
+                     static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), words16, partitions);
+  return hipGetLastError();
+}
+
+hipError_t launch_run_gather(const void* src, uint64_t words16, uint32_t part_words, uint32_t f2, uint32_t run_words,
+                             uint32_t shift, uint32_t* out, hipStream_t stream, uint32_t grid) {
+  hipLaunchKernelGGL(run_gather_kernel, dim3(grid), dim3(256), 0, stream, static_cast<const u32x4*>(src), words16,
+                     part_words, f2, run_words, shift, out);
   return hipGetLastError();
 }
 

@@ -224,5 +224,9 @@ hipError_t launch_gen_zipf(KeyType t, void* out, uint64_t n, uint64_t num_ranks,
 hipError_t launch_fill_u64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, uint64_t words16, hipStream_t stream);
+hipError_t launch_line_scatter(void* dst, const void* src, uint64_t words16, uint32_t partitions, uint32_t chunks,
+                               hipStream_t stream);
+hipError_t launch_run_gather(const void* src, uint64_t words16, uint32_t part_words, uint32_t f2, uint32_t run_words,
+                             uint32_t shift, uint32_t* out, hipStream_t stream, uint32_t grid);
 
 }  // namespace fij

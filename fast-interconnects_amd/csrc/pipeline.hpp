@@ -102,7 +102,8 @@ struct BatchSide {
 
 // Allocates one side's buffers (capacity n tuples, F1 x F2 partitions, G1
 // chunks; rel2 and meta only for two passes, meta with `tiles` + 1 rows) and
-// records every allocation in `allocs`.
+// records every allocation in `allocs`.  rel1 holds n + F1 * pad_tuples(t)
+// tuples: the plan using it must set its pad to pad_tuples(t).
 hipError_t alloc_pipeline_side(PipelineSide& s, KeyType t, uint64_t n, uint32_t F1, uint32_t F2, uint32_t G1,
                                bool two_pass, uint32_t tiles, std::vector<void*>& allocs);
 // Pass 1 of one side: contiguous histogram + offsets scan, then the scatter.

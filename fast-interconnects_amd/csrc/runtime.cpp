@@ -99,6 +99,41 @@ fij_status fij_copy_kernel(void* dst, const void* src, uint64_t bytes, void* str
   return FIJ_OK;
 }
 
+fij_status fij_calib_line_scatter(void* dst, const void* src, uint64_t bytes, uint32_t partitions, uint32_t chunks,
+                                  void* stream) {
+  if (!dst || !src) return fij::set_error(FIJ_INVALID_ARGUMENT, "NULL buffer");
+  if (bytes % 16 || reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+    return fij::set_error(FIJ_INVALID_ARGUMENT, "line scatter needs 16-byte aligned buffers and sizes");
+  const uint64_t words = bytes / 16;
+  if (chunks == 0 || partitions == 0 || words % chunks)
+    return fij::set_error(FIJ_INVALID_ARGUMENT, "line scatter: the chunks must divide the words");
+  const uint64_t chunk = words / chunks;
+  // whole groups of eight 1024-word steps per block, whole lines per piece
+  if (chunk == 0 || chunk % (1024 * 8) || (chunk / 8) % partitions)
+    return fij::set_error(FIJ_INVALID_ARGUMENT, "line scatter: a chunk must be a multiple of 8192 words and of "
+                                                "8 x partitions words");
+  RT_HIP(fij::launch_line_scatter(dst, src, words, partitions, chunks, static_cast<hipStream_t>(stream)));
+  return FIJ_OK;
+}
+
+fij_status fij_calib_run_gather(const void* src, uint64_t bytes, uint32_t part_words, uint32_t f2, uint32_t run_words,
+                                uint32_t shift, uint32_t* checksum, uint32_t grid, void* stream) {
+  if (!src || !checksum) return fij::set_error(FIJ_INVALID_ARGUMENT, "NULL buffer");
+  if (bytes % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+    return fij::set_error(FIJ_INVALID_ARGUMENT, "run gather needs a 16-byte aligned buffer and size");
+  const uint64_t words = bytes / 16;
+  if (part_words == 0 || words % part_words || f2 == 0 || part_words % f2 || grid == 0)
+    return fij::set_error(FIJ_INVALID_ARGUMENT, "run gather: partitions must divide the words, f2 a partition");
+  const uint32_t item = part_words / f2;
+  // every run stays inside its tile: shift <= run_words
+  if (item > 1024 || run_words == 0 || item % run_words || shift > run_words)
+    return fij::set_error(FIJ_INVALID_ARGUMENT, "run gather: items of at most 1024 words, whole runs, "
+                                                "shift <= run length");
+  RT_HIP(fij::launch_run_gather(src, words, part_words, f2, run_words, shift, checksum,
+                                static_cast<hipStream_t>(stream), grid));
+  return FIJ_OK;
+}
+
 fij_status fij_stream_create(void** out) {
   if (!out) return fij::set_error(FIJ_INVALID_ARGUMENT, "out is NULL");
   hipStream_t s = nullptr;

@@ -193,6 +193,8 @@ _proto("fij_memcpy_async", C.c_int, _vp, _vp, _u64, _vp)
 _proto("fij_mem_fill_u32", C.c_int, _vp, _u32, _u64, _vp)
 _proto("fij_mem_fill_u64", C.c_int, _vp, _u64, _u64, _vp)
 _proto("fij_copy_kernel", C.c_int, _vp, _vp, _u64, _vp)
+_proto("fij_calib_line_scatter", C.c_int, _vp, _vp, _u64, _u32, _u32, _vp)
+_proto("fij_calib_run_gather", C.c_int, _vp, _u64, _u32, _u32, _u32, _u32, _vp, _u32, _vp)
 _proto("fij_stream_create", C.c_int, C.POINTER(_vp))
 _proto("fij_stream_destroy", C.c_int, _vp)
 _proto("fij_stream_synchronize", C.c_int, _vp)
@@ -217,6 +219,20 @@ def copy_kernel(dst, src, nbytes=None, stream=None):
     """The library's plain device copy kernel (bandwidth reference)."""
     nbytes = src.nbytes() if nbytes is None else nbytes
     check(lib.fij_copy_kernel(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, _stream(stream)))
+
+
+def calib_line_scatter(dst, src, partitions, chunks, nbytes=None, stream=None):
+    """Calibration: the pass-1 scatter's line-write pattern as a plain copy."""
+    nbytes = src.nbytes() if nbytes is None else nbytes
+    check(lib.fij_calib_line_scatter(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, partitions, chunks,
+                                     _stream(stream)))
+
+
+def calib_run_gather(src, part_words, f2, run_words, shift, checksum, grid=512, nbytes=None, stream=None):
+    """Calibration: the register join's run gathers without table work."""
+    nbytes = src.nbytes() if nbytes is None else nbytes
+    check(lib.fij_calib_run_gather(C.c_void_p(src.ptr), nbytes, part_words, f2, run_words, shift,
+                                   C.c_void_p(checksum.ptr), grid, _stream(stream)))
 
 
 class DeviceArray:

@@ -159,6 +159,23 @@ fij_status fij_mem_fill_u64(void *ptr, uint64_t value, uint64_t count, void *str
  * measured beside the partition and join kernels (no counterpart in the
  * reference, whose measurements quote the nominal peak). */
 fij_status fij_copy_kernel(void *dst, const void *src, uint64_t bytes, void *stream);
+/* Calibration kernels, asynchronous on `stream` (no counterpart in the
+ * reference): the access patterns of the pass-1 scatter and of the join's
+ * gathers without their partitioning and table work, so that each kernel's
+ * time can be set beside what its memory pattern alone costs on this GPU.
+ * fij_calib_line_scatter copies `bytes` of 16-byte words from src to dst: block
+ * c of `chunks` reads chunk c in order and writes whole 128-byte lines
+ * round-robin to `partitions` regions, each holding the chunks' pieces in
+ * chunk order (a chunk must be a multiple of 8192 words and of 8 x partitions
+ * words).  fij_calib_run_gather reads src as partitions of part_words words,
+ * each f2 items of part_words / f2 <= 1024 words gathered by one wave as runs
+ * of run_words words (run r in tile r of the partition, tiles of f2 runs),
+ * every run but a tile's last shifted by shift <= run_words words; the xor of
+ * the loaded words goes into *checksum (device memory). */
+fij_status fij_calib_line_scatter(void *dst, const void *src, uint64_t bytes, uint32_t partitions, uint32_t chunks,
+                                  void *stream);
+fij_status fij_calib_run_gather(const void *src, uint64_t bytes, uint32_t part_words, uint32_t f2,
+                                uint32_t run_words, uint32_t shift, uint32_t *checksum, uint32_t grid, void *stream);
 fij_status fij_stream_create(void **out);
 fij_status fij_stream_destroy(void *stream);
 fij_status fij_stream_synchronize(void *stream);

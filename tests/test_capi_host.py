@@ -388,3 +388,33 @@ def test_ooc_columns_checked_before_the_call():
         fij.ooc_join(np.int64, k.astype(np.int32), k, k, k, 4, 4, pass1_cpu=True)
     with pytest.raises(ValueError):
         fij.ooc_join(np.int32, k, k, k, k, 4, 4, pass1_cpu=True)
+
+
+@pytest.mark.parametrize("bytes_, parts, chunks", [
+    (16 * (1 << 20) + 8, 512, 16),   # not whole words
+    (16 * (1 << 20), 512, 0),        # no chunks
+    (16 * (1 << 20), 512, 3),        # chunks do not divide the words
+    (16 * 4096 * 2, 512, 2),         # chunks of 4096 words: not whole groups of 8192
+])
+def test_calib_line_scatter_rejects_shapes(bytes_, parts, chunks):
+    st = fij.lib.fij_calib_line_scatter(C.c_void_p(FAKE), C.c_void_p(FAKE), bytes_, parts, chunks, None)
+    assert st == Status.INVALID_ARGUMENT
+
+
+def test_calib_line_scatter_rejects_uneven_pieces():
+    # 8192-word chunks hold 1024 lines: 3000 partitions do not divide them
+    st = fij.lib.fij_calib_line_scatter(C.c_void_p(FAKE), C.c_void_p(FAKE), 16 * 8192 * 4, 3000, 4, None)
+    assert st == Status.INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("part, f2, run, shift", [
+    (262144, 512, 16, 17),   # shift past the run: the last words would leave the tile
+    (262144, 512, 24, 1),    # runs do not divide an item
+    (262144, 128, 16, 1),    # items of 2048 words
+    (262143, 512, 16, 1),    # partitions do not divide the words
+    (262144, 0, 16, 1),
+])
+def test_calib_run_gather_rejects_shapes(part, f2, run, shift):
+    st = fij.lib.fij_calib_run_gather(C.c_void_p(FAKE), 16 * (1 << 20), part, f2, run, shift, C.c_void_p(FAKE), 512,
+                                      None)
+    assert st == Status.INVALID_ARGUMENT
