@@ -362,7 +362,7 @@ fij_join_tuning class_rule(KeyType kt, uint64_t build_len, uint64_t probe_len) {
   // join's registers (at most 640 per side), split evenly over the passes
   // (pass 1 at most 9 bits: its LDS line buffers), and enough pass-1
   // partitions that a pair's probe side has at most 64 pass-2 tiles of 8192
-  // tuples.
+  // tuples (int32 tiles hold 16,384: at most 32 of them).
   uint32_t b = std::max<uint32_t>(
       2, std::max(ceil_log2((build_len + 511) / 512), ceil_log2((probe_len + 511) / 512)));
   uint32_t b1 = std::min<uint32_t>(9, (b + 1) / 2);

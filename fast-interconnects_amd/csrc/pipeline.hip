@@ -183,6 +183,31 @@ __device__ __forceinline__ U uniform_load(const U* ptr) {
 
 // ------------------------------------------------------------- sort2 ----
 
+// Tuples per pass-2 tile: 128 KiB of LDS (16,384 8-byte or 8,192 16-byte
+// tuples).  A tile holds one run per pass-2 digit, and the join's gathers
+// of a run cost by the cache lines it touches: longer runs, fewer partial
+// lines.
+template <typename K>
+constexpr uint32_t kTile2 = 131072 / sizeof(Tuple<K>);
+
+// Raw buffer resource over `bytes` bytes at `base` (uniform): loads past the
+// end return zeros.  Third-dword flags of a raw 32-bit buffer on gfx9-family
+// parts.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+
+// Non-temporal 8-byte tuple load from a buffer resource (cache policy bit 1 = nt).
+template <typename K>
+__device__ __forceinline__ Tuple<K> buffer_load_tuple_nt(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  static_assert(sizeof(K) == 4, "8-byte tuples");
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, static_cast<int>(voff), static_cast<int>(soff), 2);
+  Tuple<K> t;
+  t.key = static_cast<K>(v[0]);
+  t.value = static_cast<K>(v[1]);
+  return t;
+}
+
 // One pass-2 tile: relation, pass-1 partition p, tile j of p, its length and
 // its first tuple in rel1.
 template <typename K>
@@ -229,24 +254,40 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     r.src = static_cast<const T*>(s.rel1) + uniform_load(s.off1 + lo) + static_cast<uint64_t>(r.j) * TILE2;
     return r;
   };
-  // lanes past the tile's end re-read its last tuple: unconditional loads
+  // Non-temporal loads.  8-byte tuples (16 per thread) use buffer loads over
+  // the tile: one offset register for all of them (the step goes in the
+  // scalar offset), which keeps two 16-tuple buffers within 128 VGPRs; lanes
+  // past the tile's end read zeros.  16-byte tuples (8 per thread) use
+  // global loads; lanes past the end re-read its last tuple.  Either way
+  // such lanes count nothing.
   auto load = [&](const TileRef<K>& r, T (&v)[ITEMS]) {
+    if constexpr (sizeof(K) == 4) {
+      const __amdgpu_buffer_rsrc_t rs = tile_rsrc(r.src, r.n * static_cast<uint32_t>(sizeof(T)));
 #pragma unroll
-    for (uint32_t k = 0; k < ITEMS; ++k) v[k] = load_tuple_nt<K>(r.src + min(k * BLOCK + tid, r.n - 1));
+      for (uint32_t k = 0; k < ITEMS; ++k)
+        v[k] = buffer_load_tuple_nt<K>(rs, tid * static_cast<uint32_t>(sizeof(T)),
+                                       k * BLOCK * static_cast<uint32_t>(sizeof(T)));
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < ITEMS; ++k) v[k] = load_tuple_nt<K>(r.src + min(k * BLOCK + tid, r.n - 1));
+    }
   };
   auto process = [&](const TileRef<K>& r, const T (&v)[ITEMS]) {
     const PipelineSide& s = r.rel ? d.probe : d.build;
-    uint32_t dig[ITEMS], rank[ITEMS];
+    // digit | rank << 16 per tuple (F2 <= 2^16, rank < TILE2 <= 2^16)
+    uint32_t dr[ITEMS];
 #pragma unroll
     for (uint32_t k = 0; k < ITEMS; ++k) {
       const bool act = k * BLOCK + tid < r.n;
-      dig[k] = key_to_partition<K>(v[k].key, mask2, d.bits1);
+      const uint32_t dig = key_to_partition<K>(v[k].key, mask2, d.bits1);
       // 8-byte tuples: one LDS atomic per tuple is faster; 16-byte tuples
       // (half as many per tile) gain from aggregating lanes of a digit
+      uint32_t rank;
       if constexpr (sizeof(K) == 4)
-        rank[k] = act ? atomicAdd(&cnt[dig[k]], 1u) : 0u;
+        rank = act ? atomicAdd(&cnt[dig], 1u) : 0u;
       else
-        rank[k] = wave_rank_add(cnt, dig[k], act);
+        rank = wave_rank_add(cnt, dig, act);
+      dr[k] = dig | (rank << 16);
     }
     __syncthreads();
     block_exclusive_scan<uint32_t>(cnt, F2, wave_tmp);
@@ -255,7 +296,7 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     for (uint32_t i = tid; i < F2; i += BLOCK) meta[i] = cnt[i] | ((i + 1 < F2 ? cnt[i + 1] : r.n) << 16);
 #pragma unroll
     for (uint32_t k = 0; k < ITEMS; ++k)
-      if (k * BLOCK + tid < r.n) sorted[cnt[dig[k]] + rank[k]] = v[k];
+      if (k * BLOCK + tid < r.n) sorted[cnt[dr[k] & 0xffffu] + (dr[k] >> 16)] = v[k];
     __syncthreads();
     T* out = static_cast<T*>(s.rel2) + uniform_load(s.pstart + r.p) + static_cast<uint64_t>(r.j) * TILE2;
     for (uint32_t i = tid; i < r.n; i += BLOCK) {
@@ -1356,7 +1397,7 @@ hipError_t launch_sum_aggregates(const JoinAggregate* src, uint32_t n, JoinAggre
   return hipGetLastError();
 }
 
-uint32_t pipeline_tile2(KeyType t) { (void)t; return 8192u; }
+uint32_t pipeline_tile2(KeyType t) { return t == KeyType::I32 ? kTile2<int32_t> : kTile2<int64_t>; }
 
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream) {
   const uint32_t blocks = d.sides == 3 ? 2u : 1u;
@@ -1382,7 +1423,7 @@ hipError_t launch_pipeline_plan_batched(const PipelineDevice& d, const BatchSide
 
 template <typename K>
 hipError_t launch_sort2_t(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
-  constexpr uint32_t TILE2 = 8192;
+  constexpr uint32_t TILE2 = kTile2<K>;
   constexpr uint32_t BLOCK = 1024;
   auto kern = sort2_kernel<K, TILE2, BLOCK>;
   const size_t lds = sort2_lds_bytes<K, TILE2>(d.F2);
@@ -1399,8 +1440,8 @@ hipError_t launch_pipeline_sort2(const PipelineDevice& d, KeyType t, uint32_t gr
 }
 
 size_t pipeline_sort2_lds(KeyType t, uint32_t F2) {
-  if (t == KeyType::I32) return sort2_lds_bytes<int32_t, 8192>(F2);
-  return sort2_lds_bytes<int64_t, 8192>(F2);
+  if (t == KeyType::I32) return sort2_lds_bytes<int32_t, kTile2<int32_t>>(F2);
+  return sort2_lds_bytes<int64_t, kTile2<int64_t>>(F2);
 }
 
 template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t BPL, uint32_t SPL>
