@@ -299,9 +299,32 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
       if (k * BLOCK + tid < r.n) sorted[cnt[dr[k] & 0xffffu] + (dr[k] >> 16)] = v[k];
     __syncthreads();
     T* out = static_cast<T*>(s.rel2) + uniform_load(s.pstart + r.p) + static_cast<uint64_t>(r.j) * TILE2;
-    for (uint32_t i = tid; i < r.n; i += BLOCK) {
-      const T x = sorted[i];
-      store_tuple_nt<K>(out + i, x.key, x.value);
+    if constexpr (sizeof(K) == 4) {
+      // 8-byte tuples: two per lane and 16-byte store, from the first
+      // 16-byte aligned tuple of the tile on (a = 0 or 1 tuple before it,
+      // and a last odd tuple, stored alone)
+      const uint32_t a = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(out) / 8) & 1u);
+      const uint32_t pairs = (r.n - min(a, r.n)) / 2;
+      for (uint32_t i = tid; i < pairs; i += BLOCK) {
+        const T x = sorted[a + 2 * i], y = sorted[a + 2 * i + 1];
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        u32x4 v;
+        v.x = static_cast<uint32_t>(x.key);
+        v.y = static_cast<uint32_t>(x.value);
+        v.z = static_cast<uint32_t>(y.key);
+        v.w = static_cast<uint32_t>(y.value);
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(out + a + 2 * i));
+      }
+      if (tid == 0 && a && r.n) store_tuple_nt<K>(out, sorted[0].key, sorted[0].value);
+      if (tid == 1 && a + 2 * pairs < r.n) {
+        const uint32_t e = a + 2 * pairs;
+        store_tuple_nt<K>(out + e, sorted[e].key, sorted[e].value);
+      }
+    } else {
+      for (uint32_t i = tid; i < r.n; i += BLOCK) {
+        const T x = sorted[i];
+        store_tuple_nt<K>(out + i, x.key, x.value);
+      }
     }
     for (uint32_t i = tid; i < F2; i += BLOCK) cnt[i] = 0;
     __syncthreads();
