@@ -448,7 +448,9 @@ def served_main(a):
     # allocated before timing (no allocation timed); beside it, with a plan
     # created and destroyed per join
     alone, t_plain, t_alloc = [], [], []
-    classes = [fij.autotune(np.int64, int(n_b[b]), int(n_p[j])) for j, (b, _, _) in enumerate(probes)]
+    # the plain launch's bits for a pair run alone (as the server chooses
+    # them: one pass where the build side's partitions fit block tables)
+    classes = [(False,) + fij.plan_bits(np.int64, int(n_b[b]), int(n_p[j])) for j, (b, _, _) in enumerate(probes)]
     caps = {}
     for (_, b1, b2), (b, _, _), j in zip(classes, probes, range(len(probes))):
         c = caps.get((b1, b2), (0, 0))

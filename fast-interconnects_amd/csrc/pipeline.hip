@@ -183,6 +183,11 @@ __device__ __forceinline__ U uniform_load(const U* ptr) {
 
 // ------------------------------------------------------------- sort2 ----
 
+// Slots of the block-shared LDS table of a deferred item (64 KiB of keys and
+// values); such an item's build side may fill 3/4 of them.
+template <typename K>
+constexpr uint32_t kBlockTableSlots = 65536 / (2 * sizeof(K));
+
 // Tuples per pass-2 tile: 128 KiB of LDS (16,384 8-byte or 8,192 16-byte
 // tuples).  A tile holds one run per pass-2 digit, and the join's gathers
 // of a run cost by the cache lines it touches: longer runs, fewer partial
@@ -1420,6 +1425,10 @@ hipError_t launch_sum_aggregates(const JoinAggregate* src, uint32_t n, JoinAggre
   return hipGetLastError();
 }
 
+uint32_t pipeline_block_table_tuples(KeyType t) {
+  return (t == KeyType::I32 ? kBlockTableSlots<int32_t> : kBlockTableSlots<int64_t>) * 3 / 4;
+}
+
 uint32_t pipeline_tile2(KeyType t) { return t == KeyType::I32 ? kTile2<int32_t> : kTile2<int64_t>; }
 
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream) {
@@ -1521,7 +1530,7 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
       return launch_register_join_sized<K, CAP, WAVES>(d, grid, stream);
   }
   // block-shared tables: 64 KB, two blocks of 512 threads per CU
-  constexpr uint32_t BCAP = 65536 / (2 * sizeof(K));
+  constexpr uint32_t BCAP = kBlockTableSlots<K>;
   constexpr uint32_t BBLOCK = 512;
   hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(kUnitsWaves * 64), 0, stream, d, BCAP * 3 / 4);
   hipError_t e = hipGetLastError();

@@ -113,6 +113,9 @@ hipError_t enqueue_side_scatter(const PipelineDevice& d, PipelineSide& s, KeyTyp
                                 uint32_t lds_total, hipStream_t stream);
 
 uint32_t pipeline_tile2(KeyType t);
+// Build tuples a deferred item may have to be joined from one block-shared
+// LDS table (3/4 of its slots).
+uint32_t pipeline_block_table_tuples(KeyType t);
 size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
 // Partition lengths, pass-2 starts and tile maps of every join side of a

@@ -407,6 +407,51 @@ fij_status register_build(fij_join_server* s, std::unique_ptr<ResidentBuild> b, 
   return FIJ_OK;
 }
 
+// One-pass resident build side: pass 1 only (bits1 bits), kept in the
+// pass-1 layout with its offsets and partition lengths.
+fij_status add_one_pass_build(fij_join_server* s, std::unique_ptr<ResidentBuild> b, const void* keys,
+                              const void* payloads, uint64_t len, uint32_t b1, uint32_t* build_id) {
+  b->bits1 = b1;
+  b->bits2 = 0;
+  b->n = len;
+  const uint32_t F1 = 1u << b1;
+  hipError_t e = alloc_pipeline_side(b->side, s->kt, len, F1, 1, s->cus, false, 0, b->allocs);
+  if (e != hipSuccess) {
+    free_all(b->allocs);
+    return hip_status(e, "hipMalloc");
+  }
+  PipelineDevice d = base_device(s, *b);
+  d.build.keys = keys;
+  d.build.pays = payloads;
+  d.build.n = len;
+  d.G1 = chunks_for(s->kt, len, s->cus);
+  d.sides = 1;
+  d.work_counter = s->scratch_counters;
+  d.aggregates = s->scratch_agg;
+  hipStream_t stm = s->setup;
+  if ((e = enqueue_side_prefix_sum(d, d.build, s->kt, 1024, stm)) != hipSuccess ||
+      (e = enqueue_side_scatter(d, d.build, s->kt, 1024, s->lds_total, stm)) != hipSuccess ||
+      (e = launch_pipeline_plan(d, stm)) != hipSuccess || (e = hipStreamSynchronize(stm)) != hipSuccess) {
+    free_all(b->allocs);
+    return hip_status(e, "partitioning the build side");
+  }
+  // the join reads rel1, off1 and plen: release the histogram and scan buffers
+  std::vector<void*> keep;
+  for (void* p : b->allocs) {
+    if (p == b->side.local1 || p == b->side.counts1 || p == b->side.scan1)
+      (void)hipFree(p);
+    else
+      keep.push_back(p);
+  }
+  b->allocs.swap(keep);
+  b->side.local1 = nullptr;
+  b->side.counts1 = nullptr;
+  b->side.scan1 = nullptr;
+  b->side.keys = nullptr;
+  b->side.pays = nullptr;
+  return register_build(s, std::move(b), build_id);
+}
+
 }  // namespace
 
 extern "C" {
@@ -450,12 +495,19 @@ fij_status fij_join_server_add_build(fij_join_server* s, const void* keys, const
     b->n = len;
     return register_build(s, std::move(b), build_id);
   }
-  // A resident build side serves probes of any length up to max_probe_len:
-  // its bits come from its own length (pairs of about 512 build tuples), with
-  // enough pass-1 partitions for the longest probe (at most 64 pass-2 tiles
-  // per partition).
-  uint32_t small = 0, b1 = 0, b2 = 0;
-  fij_status st = fij_join_autotune(static_cast<fij_type>(s->cfg.type), len, len, &small, &b1, &b2);
+  // A build side whose pass-1 partitions fit block tables is partitioned in
+  // one pass of 9 bits and kept in its pass-1 layout: a request partitions
+  // its probe side once and streams its partitions against them.
+  uint32_t b1 = 0, b2 = 0;
+  fij_status st = fij_join_plan_bits(static_cast<fij_type>(s->cfg.type), len, s->cfg.max_probe_len, &b1, &b2);
+  if (st != FIJ_OK) return st;
+  if (b2 == 0) return add_one_pass_build(s, std::move(b), keys, payloads, len, b1, build_id);
+  // Otherwise the build side serves probes of any length up to
+  // max_probe_len: its bits come from its own length (pairs of about 512
+  // build tuples), with enough pass-1 partitions for the longest probe (at
+  // most 64 pass-2 tiles per partition).
+  uint32_t small = 0;
+  st = fij_join_autotune(static_cast<fij_type>(s->cfg.type), len, len, &small, &b1, &b2);
   if (st != FIJ_OK) return st;
   {
     uint64_t mp = s->cfg.max_probe_len ? s->cfg.max_probe_len - 1 : 0;

@@ -855,6 +855,18 @@ _proto("fij_join_batch_execute", C.c_int, _vp, _u32, C.POINTER(JoinDesc), _vp, C
 _proto("fij_join_batch_device_bytes", C.c_int, _vp, C.POINTER(_u64))
 
 
+_proto("fij_join_plan_bits", C.c_int, C.c_int, _u64, _u64, C.POINTER(_u32), C.POINTER(_u32))
+
+
+def plan_bits(dtype, build_len, probe_len):
+    """(radix_bits_1, radix_bits_2) for one join run alone: a single pass of
+    9 bits when the build side's partitions fit block tables, else the
+    size-class rule's two passes."""
+    b1, b2 = _u32(), _u32()
+    check(lib.fij_join_plan_bits(int(_ktype(dtype)), build_len, probe_len, C.byref(b1), C.byref(b2)))
+    return b1.value, b2.value
+
+
 def autotune(dtype, build_len, probe_len):
     """(small, radix_bits_1, radix_bits_2) of the pair's size class (tuning
     table entry, or the closed-form rule)."""

@@ -362,6 +362,14 @@ typedef struct fij_join_timing {
   float deferred_ms;
 } fij_join_timing;
 
+/* Radix bits for one join run alone (a plan, or a served request): one
+ * pass of 9 bits when the build side's pass-1 partitions hold on average at
+ * most 2/3 of a block-shared LDS table (|R| <= 2^20 at int64, 2^21 at
+ * int32), so that the probe side is partitioned once and its partitions are
+ * streamed against them; otherwise the two-pass bits of fij_join_autotune.
+ * No counterpart in the reference, whose caller chooses the bits. */
+fij_status fij_join_plan_bits(fij_type type, uint64_t build_len, uint64_t probe_len, uint32_t *radix_bits_1,
+                              uint32_t *radix_bits_2);
 fij_status fij_join_plan_create(const fij_join_plan_config *config, fij_join_plan **out);
 fij_status fij_join_plan_destroy(fij_join_plan *plan);
 /* Device bytes the plan holds (partitions, offsets, tile metadata). */

@@ -418,3 +418,17 @@ def test_calib_run_gather_rejects_shapes(part, f2, run, shift):
     st = fij.lib.fij_calib_run_gather(C.c_void_p(FAKE), 16 * (1 << 20), part, f2, run, shift, C.c_void_p(FAKE), 512,
                                       None)
     assert st == Status.INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("dtype, one_pass_max", [(np.int64, 1 << 20), (np.int32, 1 << 21)])
+def test_plan_bits_one_pass_up_to_block_tables(dtype, one_pass_max):
+    """One pass of 9 bits while |R| / 512 stays within 2/3 of a block table
+    (3,072 int64 or 6,144 int32 build tuples); beyond it the size-class
+    rule's two passes."""
+    assert fij.plan_bits(dtype, one_pass_max, 1 << 28) == (9, 0)
+    assert fij.plan_bits(dtype, 1000, 1 << 14) == (9, 0)
+    b1, b2 = fij.plan_bits(dtype, one_pass_max + 1, 1 << 28)
+    _, r1, r2 = fij.autotune(dtype, one_pass_max + 1, 1 << 28)
+    assert (b1, b2) == (r1, r2) and b2 > 0
+    with pytest.raises(FijError):
+        fij.check(fij.lib.fij_join_plan_bits(7, 10, 10, C.byref(C.c_uint32()), C.byref(C.c_uint32())))
