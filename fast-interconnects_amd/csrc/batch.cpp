@@ -246,8 +246,10 @@ fij_status enqueue_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, c
       l.rel1_at = rel1[s];
       rel1[s] += (l.n + static_cast<uint64_t>(F1) * pad + L - 1) / L * L;  // regions start on 128-byte lines
       l.rel2_at = rel2[s];
-      rel2[s] += l.n;
-      tiles[s] += (l.n + tile2 - 1) / tile2 + F1;  // at most
+      if (b2) {  // single-pass classes join from rel1
+        rel2[s] += l.n;
+        tiles[s] += (l.n + tile2 - 1) / tile2 + F1;  // at most
+      }
       blocks += l.chunks;
     }
     probe_total += jd.probe_len;
@@ -339,14 +341,19 @@ hipError_t launch_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, hi
   if ((e = launch_pipeline_plan_batched(a.d, a.sides, a.sub_joins, a.side_tiles, a.d.build.n, a.d.probe.n, st)) !=
       hipSuccess)
     return e;
-  const uint64_t tiles = static_cast<uint64_t>(a.sub_tiles[0]) + a.sub_tiles[1];
-  const uint32_t sort_grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(b->cus, tiles)));
-  if ((e = launch_pipeline_sort2(a.d, kt, sort_grid, st)) != hipSuccess) return e;
+  const bool one_pass = ck.b2 == 0;
+  if (!one_pass) {
+    const uint64_t tiles = static_cast<uint64_t>(a.sub_tiles[0]) + a.sub_tiles[1];
+    const uint32_t sort_grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(b->cus, tiles)));
+    if ((e = launch_pipeline_sort2(a.d, kt, sort_grid, st)) != hipSuccess) return e;
+  }
   // the register join: four blocks per CU, as many as the lighter join shapes
   // of batched classes (fewer registers, smaller tables) keep resident; the
-  // others start as earlier blocks finish and find the items taken
-  if ((e = launch_pipeline_join(a.d, kt, false, false, 4 * b->cus, st)) != hipSuccess) return e;
-  return launch_pipeline_join(a.d, kt, false, true, 2 * b->cus, st);
+  // others start as earlier blocks finish and find the items taken.  A
+  // single-pass class's items are whole pass-1 partitions (large tables);
+  // most go on to the block-table join.
+  if ((e = launch_pipeline_join(a.d, kt, one_pass, false, 4 * b->cus, st)) != hipSuccess) return e;
+  return launch_pipeline_join(a.d, kt, one_pass, true, 2 * b->cus, st);
 }
 
 // The closed-form size-class rule (the built-in table's fallback).
@@ -394,10 +401,12 @@ std::vector<fij_join_tuning> builtin_table(KeyType kt) {
 std::mutex g_tuning_mutex;
 std::vector<fij_join_tuning> g_tuning[2];
 bool g_tuning_loaded[2] = {false, false};
+bool g_tuning_user[2] = {false, false};  // the table came from fij_join_tuning_set entries
 
-// The configuration of a pair of these lengths: the table entry of its class
-// when there is one (the small path only where the pair qualifies for it),
-// otherwise the rule.
+// The configuration of a pair of these lengths: an entry of a table set by
+// fij_join_tuning_set; else a single-pass plan where it applies; else the
+// built-in table's entry of its class (the small path only where the pair
+// qualifies for it); else the rule.
 fij_join_tuning tuned_class(KeyType kt, uint64_t build_len, uint64_t probe_len) {
   const fij_join_tuning rule = class_rule(kt, build_len, probe_len);
   const int ti = kt == KeyType::I32 ? 0 : 1;
@@ -405,11 +414,30 @@ fij_join_tuning tuned_class(KeyType kt, uint64_t build_len, uint64_t probe_len) 
   if (!g_tuning_loaded[ti]) {
     g_tuning[ti] = builtin_table(kt);
     g_tuning_loaded[ti] = true;
+    g_tuning_user[ti] = false;
   }
-  for (const fij_join_tuning& e : g_tuning[ti]) {
-    if (e.build_log2 != rule.build_log2 || e.probe_log2 != rule.probe_log2) continue;
-    fij_join_tuning t = e;
-    t.small = e.small && rule.small;
+  const fij_join_tuning* entry = nullptr;
+  for (const fij_join_tuning& e : g_tuning[ti])
+    if (e.build_log2 == rule.build_log2 && e.probe_log2 == rule.probe_log2) {
+      entry = &e;
+      break;
+    }
+  // Build sides of 2^18 tuples up to what block tables hold after one pass
+  // of 9 bits (fij_join_plan_bits): a single-pass plan, measured faster than
+  // the two-pass classes there (1.2-1.45x for single int32 joins at
+  // 2^19-2^21, up to 2.9x for long probe sides at int64; batched int32 joins
+  // of log-uniform sizes 6 % faster); below 2^18 the two are level.
+  if (!(entry && g_tuning_user[ti]) && !rule.small && build_len >= (1u << 18) && one_pass_fits(kt, build_len)) {
+    fij_join_tuning t = rule;
+    t.radix_bits_1 = kOnePassBits;
+    t.radix_bits_2 = 0;
+    t.ring_lines = 0;
+    t.block = 1024;
+    return t;
+  }
+  if (entry) {
+    fij_join_tuning t = *entry;
+    t.small = entry->small && rule.small;
     t.from_table = 1;
     return t;
   }
@@ -461,6 +489,7 @@ fij_status fij_join_tuning_set(fij_type type, const fij_join_tuning* entries, ui
   std::lock_guard<std::mutex> lock(g_tuning_mutex);
   g_tuning[type == FIJ_TYPE_I32 ? 0 : 1] = std::move(table);
   g_tuning_loaded[type == FIJ_TYPE_I32 ? 0 : 1] = true;
+  g_tuning_user[type == FIJ_TYPE_I32 ? 0 : 1] = !use_builtin;
   return FIJ_OK;
 }
 

@@ -116,6 +116,15 @@ uint32_t pipeline_tile2(KeyType t);
 // Build tuples a deferred item may have to be joined from one block-shared
 // LDS table (3/4 of its slots).
 uint32_t pipeline_block_table_tuples(KeyType t);
+
+// Single-pass plans (fij_join_plan_bits, single-pass batched classes): 9
+// bits, the widest pass-1 fanout whose line rings fit LDS.
+constexpr uint32_t kOnePassBits = 9;
+// Whether a build side's pass-1 partitions at kOnePassBits hold on average
+// at most 2/3 of a block table (half again of headroom for uneven ones).
+inline bool one_pass_fits(KeyType t, uint64_t build_len) {
+  return build_len * 3 <= static_cast<uint64_t>(pipeline_block_table_tuples(t)) * (1ull << kOnePassBits) * 2;
+}
 size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);
 // Partition lengths, pass-2 starts and tile maps of every join side of a

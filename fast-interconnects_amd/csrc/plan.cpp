@@ -49,11 +49,6 @@ fij_status plan_fail(fij_status s, const char* fmt, ...) {
   return set_error(s, buf);
 }
 
-// Single-pass plans of fij_join_plan_bits: 9 bits, the widest pass-1
-// fanout whose line rings fit LDS.
-constexpr uint32_t kOnePassBits = 9;
-constexpr uint64_t kOnePassPartitions = 1ull << kOnePassBits;
-
 fij_status hip_fail(hipError_t e, const char* what) {
   return plan_fail(e == hipErrorOutOfMemory ? FIJ_OUT_OF_MEMORY : FIJ_DEVICE_ERROR, "%s: %s", what,
                    hipGetErrorString(e));
@@ -97,9 +92,7 @@ fij_status fij_join_plan_bits(fij_type type, uint64_t build_len, uint64_t probe_
   if (!radix_bits_1 || !radix_bits_2) return plan_fail(FIJ_INVALID_ARGUMENT, "invalid argument");
   if (type != FIJ_TYPE_I32 && type != FIJ_TYPE_I64) return plan_fail(FIJ_INVALID_ARGUMENT, "unknown type");
   const KeyType kt = type == FIJ_TYPE_I32 ? KeyType::I32 : KeyType::I64;
-  // one pass of 9 bits: a build side whose partitions hold on average at
-  // most 2/3 of a block table (half again of headroom for uneven ones)
-  if (build_len * 3 <= static_cast<uint64_t>(pipeline_block_table_tuples(kt)) * kOnePassPartitions * 2) {
+  if (one_pass_fits(kt, build_len)) {
     *radix_bits_1 = kOnePassBits;
     *radix_bits_2 = 0;
     return FIJ_OK;

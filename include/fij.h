@@ -398,7 +398,9 @@ fij_status fij_join_plan_execute(fij_join_plan *plan, const void *build_keys, co
  * reference, which joins one pair per run of the radix-join binary).  Joins
  * whose build side fits one block's LDS table are joined together by one
  * persistent launch without partitioning; every other join is partitioned
- * in two passes with the radix bits of its size class (fij_join_autotune),
+ * in one or two passes with the radix bits of its size class
+ * (fij_join_autotune: one pass of 9 bits for build sides from 2^18 tuples up
+ * to what block tables hold after it, as fij_join_plan_bits),
  * all joins of a class together: one launch per pass (histograms, offsets,
  * scatter, pass 2, join) covers every join of the class.  The submission is
  * recorded into one HIP graph when use_graph is set.  The size classes are

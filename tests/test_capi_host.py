@@ -302,7 +302,12 @@ def test_autotune_rule(rule_only):
     assert fij.autotune(np.int64, 6144, 1 << 20)[0] and not fij.autotune(np.int64, 6145, 1 << 20)[0]
     small, b1, b2 = fij.autotune(np.int64, 1 << 27, 1 << 27)
     assert not small and (b1, b2) == (9, 9)
-    for n_r in (1 << 14, 1 << 18, 3_000_000, 1 << 22, 1 << 26):
+    # build sides from 2^18 tuples up to the single-pass limit: one pass of 9 bits
+    for dtype, top in ((np.int32, 1 << 21), (np.int64, 1 << 20)):
+        for n_r in (1 << 18, 300000, top):
+            assert fij.autotune(dtype, n_r, 8 * n_r) == (False, 9, 0)
+        assert fij.autotune(dtype, top + 1, 8 * top)[2] > 0
+    for n_r in (1 << 14, (1 << 18) - 1, 3_000_000, 1 << 22, 1 << 26):
         for f in (1, 4, 16):
             small, b1, b2 = fij.autotune(np.int32, n_r, n_r * f)
             assert 1 <= b1 <= 9 and 1 <= b2 <= 12
@@ -329,6 +334,10 @@ def test_builtin_tuning_table():
             assert 1 <= b1 <= 9 and 1 <= b2 <= 12 and block in (512, 1024) and ring in (0, 1, 2, 4, 8, 16, 32, 64, 128, 256)
             n_r, n_s = (3 << bl) // 4, (3 << pl) // 4  # lengths inside the class
             t = fij.tuning_lookup(dtype, n_r, n_s)
+            if not small and n_r >= 1 << 18 and fij.plan_bits(dtype, n_r, n_s)[1] == 0:
+                # single-pass classes take precedence over the measured two-pass entries
+                assert (t["radix_bits_1"], t["radix_bits_2"], t["from_table"]) == (9, 0, 0)
+                continue
             assert t["from_table"] == 1 and (t["build_log2"], t["probe_log2"]) == (bl, pl)
             assert (t["radix_bits_1"], t["radix_bits_2"], t["ring_lines"], t["block"]) == (b1, b2, ring, block)
             assert t["small"] == small

@@ -54,6 +54,26 @@ def test_batch_equals_plain_plan():
         assert g == want
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_single_pass_classes(dtype):
+    """Build sides from 2^18 tuples up to what block tables hold after one
+    pass run as single-pass classes (fij_join_plan_bits), beside two-pass and
+    small joins in the same submission; duplicates and null keys included."""
+    one_pass_max = (1 << 21) if dtype == np.int32 else (1 << 20)
+    sizes = [1 << 18, 300001, one_pass_max, one_pass_max + 1, 20000, 3000]
+    host = [make_join(i, n, 2 + i % 3, dtype, seed=2000) for i, n in enumerate(sizes)]
+    rk = orc.gen_uniform(400000, 77, 1, -3, 150000, dtype)  # duplicates and null keys
+    host.append((rk, orc.gen_iota(400000, 1, dtype), orc.gen_uniform(800000, 77, 2, -3, 150000, dtype),
+                 orc.gen_iota(800000, 1, dtype)))
+    classes = [fij.autotune(dtype, len(h[0]), len(h[2])) for h in host]
+    assert [c[2] == 0 for c in classes] == [True, True, True, False, False, False, True]
+    dev = [tuple(to_dev(a) for a in j) for j in host]
+    got, stats = fij.JoinBatch(dtype, len(host), lanes=2).execute(dev)
+    for j, (h, g) in enumerate(zip(host, got)):
+        assert g == orc.sort_merge_join(*h), f"join {j} ({len(h[0])} x {len(h[2])})"
+    assert fij.device_status() == 0
+
+
 def test_batch_null_keys_and_duplicates():
     dtype = np.int32
     host = []
