@@ -297,10 +297,14 @@ def batched_main(a):
     classes = [fij.autotune(np.int32, int(r), int(s_)) for r, s_ in zip(n_r, n_s)]
     b_alg = sum((int(r) + int(s_)) * (8 if small else 4 + (2 * (2 if b2 else 1) + 1) * 8)
                 for (small, _, b2), r, s_ in zip(classes, n_r, n_s))
-    # the plain per-join launch: one plan per size class, sized for the
-    # class's largest join, allocated before any timing
+    # the plain per-join launch: each join through a plan of the bits a
+    # join run alone takes (fij_join_plan_bits: one pass where the build
+    # side's partitions fit block tables, else its size class's two passes),
+    # one plan per bits, sized for its largest join, allocated before any
+    # timing
+    plain_bits = [fij.plan_bits(np.int32, int(r), int(s_)) for r, s_ in zip(n_r, n_s)]
     pooled = {}
-    for (small, b1, b2), r, s_ in zip(classes, n_r, n_s):
+    for (b1, b2), r, s_ in zip(plain_bits, n_r, n_s):
         cap = pooled.get((b1, b2), (0, 0))
         pooled[(b1, b2)] = (max(cap[0], int(r)), max(cap[1], int(s_)))
     pooled = {key: fij.JoinPlan(fij.KeyType.I32, r, s_, *key) for key, (r, s_) in pooled.items()}
@@ -346,7 +350,7 @@ def batched_main(a):
             assert g[0] == s_ and g[2] == s_ * (s_ - 1) // 2, "batched join result"
         # the plain per-join launch, one join at a time, every join
         for j in range(0 if a.no_plain else len(joins)):
-            _, b1, b2 = classes[w[j]]
+            b1, b2 = plain_bits[w[j]]
             fij.synchronize()
             t0 = time.perf_counter()
             pooled[(b1, b2)].enqueue_sized(*joins[j], res_dev)
@@ -358,7 +362,7 @@ def batched_main(a):
         # with a plan allocated per join, on a sample
         for j in range(0, 0 if a.no_plain else len(joins), plain_step):
             r, s_ = lens[j]
-            _, b1, b2 = classes[w[j]]
+            b1, b2 = plain_bits[w[j]]
             t0 = time.perf_counter()
             plan = fij.JoinPlan(fij.KeyType.I32, r, s_, b1, b2)
             g, _ = plan.execute(*joins[j])

@@ -319,6 +319,9 @@ fij_status enqueue_class(fij_join_batch* b, ClassArena& a, const ClassKey& ck, c
   d.job_shift = b1;
   d.build.n = rel2[0];
   d.probe.n = rel2[1];
+  d.big_block_tables = 0;
+  if (b2 == 0)
+    for (uint32_t j = 0; j < J; ++j) d.big_block_tables |= big_block_tables_for(kt, lay[2 * j].n, F1);
   a.sub_joins = J;
   a.sub_blocks = blocks;
   a.sub_tiles[0] = static_cast<uint32_t>(tiles[0]);
@@ -422,12 +425,16 @@ fij_join_tuning tuned_class(KeyType kt, uint64_t build_len, uint64_t probe_len) 
       entry = &e;
       break;
     }
-  // Build sides of 2^18 tuples up to what block tables hold after one pass
-  // of 9 bits (fij_join_plan_bits): a single-pass plan, measured faster than
-  // the two-pass classes there (1.2-1.45x for single int32 joins at
-  // 2^19-2^21, up to 2.9x for long probe sides at int64; batched int32 joins
-  // of log-uniform sizes 6 % faster); below 2^18 the two are level.
-  if (!(entry && g_tuning_user[ti]) && !rule.small && build_len >= (1u << 18) && one_pass_fits(kt, build_len)) {
+  // Build sides of 2^18 tuples up to what 64 KiB block tables hold after one
+  // pass of 9 bits: a single-pass plan, measured faster than the two-pass
+  // classes there (1.2-1.45x for single int32 joins at 2^19-2^21, up to 2.9x
+  // for long probe sides at int64; batched int32 joins of log-uniform sizes
+  // 6 % faster); below 2^18 the two are level.  Going on to the 128 KiB
+  // tables that single joins take up to twice that size (fij_join_plan_bits)
+  // left the batched mix as fast (0.238 s either way on one box), so batched
+  // classes stop here.
+  if (!(entry && g_tuning_user[ti]) && !rule.small && build_len >= (1u << 18) &&
+      block_tables_hold(kt, build_len, 1ull << kOnePassBits, false)) {
     fij_join_tuning t = rule;
     t.radix_bits_1 = kOnePassBits;
     t.radix_bits_2 = 0;

@@ -184,7 +184,8 @@ __device__ __forceinline__ U uniform_load(const U* ptr) {
 // ------------------------------------------------------------- sort2 ----
 
 // Slots of the block-shared LDS table of a deferred item (64 KiB of keys and
-// values); such an item's build side may fill 3/4 of them.
+// values; twice that in single-pass plans with big_block_tables); such an
+// item's build side may fill 3/4 of them.
 template <typename K>
 constexpr uint32_t kBlockTableSlots = 65536 / (2 * sizeof(K));
 
@@ -1425,8 +1426,8 @@ hipError_t launch_sum_aggregates(const JoinAggregate* src, uint32_t n, JoinAggre
   return hipGetLastError();
 }
 
-uint32_t pipeline_block_table_tuples(KeyType t) {
-  return (t == KeyType::I32 ? kBlockTableSlots<int32_t> : kBlockTableSlots<int64_t>) * 3 / 4;
+uint32_t pipeline_block_table_tuples(KeyType t, bool big) {
+  return (t == KeyType::I32 ? kBlockTableSlots<int32_t> : kBlockTableSlots<int64_t>) * (big ? 2 : 1) * 3 / 4;
 }
 
 uint32_t pipeline_tile2(KeyType t) { return t == KeyType::I32 ? kTile2<int32_t> : kTile2<int64_t>; }
@@ -1520,17 +1521,11 @@ hipError_t launch_register_join_sized(const PipelineDevice& d, uint32_t grid, hi
   return launch_register_join<K, CAP / 2, WAVES, 4, 4>(d, grid, stream);
 }
 
-template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t GCAP, uint32_t GWAVES>
-hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, hipStream_t stream) {
-  if (!deferred) {
-    // large tables (single-pass plans): one shape
-    if constexpr (CAP > 1024)
-      return launch_register_join<K, CAP, WAVES, 10, 10>(d, grid, stream);
-    else
-      return launch_register_join_sized<K, CAP, WAVES>(d, grid, stream);
-  }
-  // block-shared tables: 64 KB, two blocks of 512 threads per CU
-  constexpr uint32_t BCAP = kBlockTableSlots<K>;
+// The deferred items' joins with block-shared tables of BCAP slots: 64 KiB
+// (two blocks of 512 threads per CU) or, for single-pass plans whose build
+// partitions need it, 128 KiB (one block per CU).
+template <typename K, uint32_t BCAP, uint32_t GCAP, uint32_t GWAVES>
+hipError_t launch_deferred_t(const PipelineDevice& d, uint32_t grid, hipStream_t stream) {
   constexpr uint32_t BBLOCK = 512;
   hipLaunchKernelGGL(join_units_kernel, dim3(grid), dim3(kUnitsWaves * 64), 0, stream, d, BCAP * 3 / 4);
   hipError_t e = hipGetLastError();
@@ -1560,6 +1555,21 @@ hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, 
   hipLaunchKernelGGL(join_hbm_clear_kernel, dim3(grid), dim3(256), 0, stream, d,
                      static_cast<uint32_t>(sizeof(Tuple<K>)));
   return hipGetLastError();
+}
+
+template <typename K, uint32_t CAP, uint32_t WAVES, uint32_t GCAP, uint32_t GWAVES>
+hipError_t launch_join_t(const PipelineDevice& d, bool deferred, uint32_t grid, hipStream_t stream) {
+  if (!deferred) {
+    // large tables (single-pass plans): one shape
+    if constexpr (CAP > 1024)
+      return launch_register_join<K, CAP, WAVES, 10, 10>(d, grid, stream);
+    else
+      return launch_register_join_sized<K, CAP, WAVES>(d, grid, stream);
+  }
+  if constexpr (CAP > 1024) {
+    if (d.big_block_tables) return launch_deferred_t<K, 2 * kBlockTableSlots<K>, GCAP, GWAVES>(d, grid, stream);
+  }
+  return launch_deferred_t<K, kBlockTableSlots<K>, GCAP, GWAVES>(d, grid, stream);
 }
 
 hipError_t launch_pipeline_join(const PipelineDevice& d, KeyType t, bool large_tables, bool deferred, uint32_t grid,

@@ -89,6 +89,9 @@ struct PipelineDevice {
   uint4* hbm_units;           // [hbm_unit_cap] {big index, first run, end run, 0}: build units from the front,
                               // probe units from the back
   uint32_t hbm_unit_cap;
+  // single-pass plans: deferred items joined from 128 KiB block tables (one
+  // block per CU) instead of 64 KiB ones (build partitions beyond the latter)
+  uint32_t big_block_tables;
 };
 
 // One join side of a batched plan: its pass-1 region and where its
@@ -114,16 +117,27 @@ hipError_t enqueue_side_scatter(const PipelineDevice& d, PipelineSide& s, KeyTyp
 
 uint32_t pipeline_tile2(KeyType t);
 // Build tuples a deferred item may have to be joined from one block-shared
-// LDS table (3/4 of its slots).
-uint32_t pipeline_block_table_tuples(KeyType t);
+// LDS table (3/4 of its slots): the 64 KiB table, or the 128 KiB one of
+// single-pass plans with big_block_tables.
+uint32_t pipeline_block_table_tuples(KeyType t, bool big = false);
 
 // Single-pass plans (fij_join_plan_bits, single-pass batched classes): 9
 // bits, the widest pass-1 fanout whose line rings fit LDS.
 constexpr uint32_t kOnePassBits = 9;
-// Whether a build side's pass-1 partitions at kOnePassBits hold on average
-// at most 2/3 of a block table (half again of headroom for uneven ones).
+// Whether a build side's single-pass partitions hold on average at most 2/3
+// of a (big) block table (half again of headroom for uneven ones).
+inline bool block_tables_hold(KeyType t, uint64_t build_len, uint64_t partitions, bool big) {
+  return build_len * 3 <= static_cast<uint64_t>(pipeline_block_table_tuples(t, big)) * partitions * 2;
+}
+// Single pass of kOnePassBits: with 128 KiB tables where 64 KiB ones do not
+// hold the partitions (the big ones run one block per CU and are slower for
+// build sides the small ones hold).
 inline bool one_pass_fits(KeyType t, uint64_t build_len) {
-  return build_len * 3 <= static_cast<uint64_t>(pipeline_block_table_tuples(t)) * (1ull << kOnePassBits) * 2;
+  return block_tables_hold(t, build_len, 1ull << kOnePassBits, true);
+}
+// big_block_tables of a single-pass plan of `partitions` partitions.
+inline uint32_t big_block_tables_for(KeyType t, uint64_t build_len, uint64_t partitions) {
+  return block_tables_hold(t, build_len, partitions, false) ? 0u : 1u;
 }
 size_t pipeline_sort2_lds(KeyType t, uint32_t F2);
 hipError_t launch_pipeline_plan(const PipelineDevice& d, hipStream_t stream);

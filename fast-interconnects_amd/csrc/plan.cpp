@@ -268,6 +268,7 @@ fij_status enqueue_join(fij_join_plan* p, const void* build_keys, const void* bu
   d.build.padded_len1 = n_b + static_cast<uint64_t>(d.F1) * d.pad;
   d.probe.padded_len1 = n_p + static_cast<uint64_t>(d.F1) * d.pad;
   d.G1 = exec_chunks(p, n_b, n_p);
+  d.big_block_tables = d.bits2 ? 0u : big_block_tables_for(p->kt, n_b, d.F1);
   if (events) PLAN_HIP(hipEventRecord(p->ev[0], st));
   for (PipelineSide* s : {&d.build, &d.probe}) PLAN_HIP(enqueue_side_prefix_sum(d, *s, p->kt, p->block1, st));
   if (events) PLAN_HIP(hipEventRecord(p->ev[1], st));

@@ -195,6 +195,7 @@ fij_status launch_request(fij_join_server* s, Slot& sl, const Request& r) {
   d.probe.padded_len1 = r.len + static_cast<uint64_t>(d.F1) * d.pad;
   d.probe.scan1_len = scan_state_words(static_cast<uint64_t>(d.F1) * s->cus);
   d.G1 = chunks_for(s->kt, r.len, s->cus);
+  d.big_block_tables = d.bits2 ? 0u : big_block_tables_for(s->kt, b.n, d.F1);
   d.sides = 2;
   d.work_counter = sl.work_counter;
   d.overflow = sl.overflow;

@@ -364,9 +364,10 @@ typedef struct fij_join_timing {
 
 /* Radix bits for one join run alone (a plan, or a served request): one
  * pass of 9 bits when the build side's pass-1 partitions hold on average at
- * most 2/3 of a block-shared LDS table (|R| <= 2^20 at int64, 2^21 at
- * int32), so that the probe side is partitioned once and its partitions are
- * streamed against them; otherwise the two-pass bits of fij_join_autotune.
+ * most 2/3 of a 128 KiB block-shared LDS table (|R| <= 2^21 at int64, 2^22
+ * at int32; plans whose partitions also fit 64 KiB tables use those), so
+ * that the probe side is partitioned once and its partitions are streamed
+ * against them; otherwise the two-pass bits of fij_join_autotune.
  * No counterpart in the reference, whose caller chooses the bits. */
 fij_status fij_join_plan_bits(fij_type type, uint64_t build_len, uint64_t probe_len, uint32_t *radix_bits_1,
                               uint32_t *radix_bits_2);
@@ -400,7 +401,7 @@ fij_status fij_join_plan_execute(fij_join_plan *plan, const void *build_keys, co
  * persistent launch without partitioning; every other join is partitioned
  * in one or two passes with the radix bits of its size class
  * (fij_join_autotune: one pass of 9 bits for build sides from 2^18 tuples up
- * to what block tables hold after it, as fij_join_plan_bits),
+ * to what 64 KiB block tables hold after it: 2^20 int64, 2^21 int32),
  * all joins of a class together: one launch per pass (histograms, offsets,
  * scatter, pass 2, join) covers every join of the class.  The submission is
  * recorded into one HIP graph when use_graph is set.  The size classes are

@@ -58,7 +58,9 @@ def test_batch_equals_plain_plan():
 def test_single_pass_classes(dtype):
     """Build sides from 2^18 tuples up to what block tables hold after one
     pass run as single-pass classes (fij_join_plan_bits), beside two-pass and
-    small joins in the same submission; duplicates and null keys included."""
+    small joins in the same submission; duplicates and null keys included.
+    Batched classes stop where 64 KiB block tables stop holding the build
+    partitions."""
     one_pass_max = (1 << 21) if dtype == np.int32 else (1 << 20)
     sizes = [1 << 18, 300001, one_pass_max, one_pass_max + 1, 20000, 3000]
     host = [make_join(i, n, 2 + i % 3, dtype, seed=2000) for i, n in enumerate(sizes)]

@@ -36,13 +36,14 @@ def make_probe(j, n_b, n, dtype):
 def test_server_alone_equals_mix_equals_oracle(dtype, table_build_max):
     """table_build_max = 2^17: the two smaller builds (one with duplicates
     and null keys) are resident hash tables, the larger ones partitioned, in
-    the same mix.  Builds of at most 2^20 (int64) or 2^21 (int32) tuples that
-    are not tables are partitioned in one pass (fij_join_plan_bits), the
-    2^21 + 4097-tuple build in two."""
+    the same mix.  Builds of at most 2^21 (int64) or 2^22 (int32) tuples that
+    are not tables are partitioned in one pass (fij_join_plan_bits; from 2^20
+    or 2^21 on with 128 KiB block tables), the larger build in two."""
+    big = (1 << (22 if dtype == np.int32 else 21)) + 4097
     builds = [make_build(0, 4096, dtype), make_build(1, 1 << 17, dtype, dup=True), make_build(2, 300000, dtype),
-              make_build(3, 0, dtype), make_build(4, (1 << 21) + 4097, dtype)]
-    assert fij.plan_bits(dtype, 300000, 1 << 20)[1] == 0
-    assert fij.plan_bits(dtype, (1 << 21) + 4097, 1 << 20)[1] > 0
+              make_build(3, 0, dtype), make_build(4, big, dtype), make_build(5, big - 8192, dtype)]
+    assert fij.plan_bits(dtype, 300000, 1 << 20)[1] == 0 and fij.plan_bits(dtype, big - 8192, 1 << 20)[1] == 0
+    assert fij.plan_bits(dtype, big, 1 << 20)[1] > 0
     srv = fij.JoinServer(dtype, [(3, 1 << 16), (2, 1 << 20)], max_probe_len=1 << 20, table_build_max=table_build_max)
     dev_builds = [tuple(to_dev(a) if len(a) else fij.DeviceArray(1, dtype).view(0, 0) for a in b) for b in builds]
     ids = [srv.add_build(*b) for b in dev_builds]
