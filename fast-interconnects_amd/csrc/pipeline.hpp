@@ -135,6 +135,17 @@ inline bool block_tables_hold(KeyType t, uint64_t build_len, uint64_t partitions
 inline bool one_pass_fits(KeyType t, uint64_t build_len) {
   return block_tables_hold(t, build_len, 1ull << kOnePassBits, true);
 }
+// Pass-2 bits of a two-pass plan of 9 + b2 bits whose items hold about a
+// third of a 64 KiB block table's build tuples (1,024 int64, 2,048 int32):
+// for probe sides far longer than the build side, whose items go to the
+// block join anyway, fewer pass-2 digits give longer runs.
+inline uint32_t probe_heavy_bits2(KeyType t, uint64_t build_len) {
+  const uint64_t per_item = pipeline_block_table_tuples(t, false) / 3;
+  const uint64_t items = (build_len >> kOnePassBits) / per_item;
+  uint32_t b2 = 1;
+  while (b2 < 12 && (1ull << b2) < items) ++b2;
+  return b2;
+}
 // big_block_tables of a single-pass plan of `partitions` partitions.
 inline uint32_t big_block_tables_for(KeyType t, uint64_t build_len, uint64_t partitions) {
   return block_tables_hold(t, build_len, partitions, false) ? 0u : 1u;

@@ -97,6 +97,13 @@ fij_status fij_join_plan_bits(fij_type type, uint64_t build_len, uint64_t probe_
     *radix_bits_2 = 0;
     return FIJ_OK;
   }
+  // a probe side 16 times the build side or longer: items of about a third
+  // of a block table, joined from block tables over long runs
+  if (probe_len >= 16 * build_len) {
+    *radix_bits_1 = kOnePassBits;
+    *radix_bits_2 = probe_heavy_bits2(kt, build_len);
+    return FIJ_OK;
+  }
   uint32_t small = 0;
   return fij_join_autotune(type, build_len, probe_len, &small, radix_bits_1, radix_bits_2);
 }

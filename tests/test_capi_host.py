@@ -432,12 +432,16 @@ def test_calib_run_gather_rejects_shapes(part, f2, run, shift):
 @pytest.mark.parametrize("dtype, one_pass_max", [(np.int64, 1 << 21), (np.int32, 1 << 22)])
 def test_plan_bits_one_pass_up_to_block_tables(dtype, one_pass_max):
     """One pass of 9 bits while |R| / 512 stays within 2/3 of a 128 KiB block
-    table (6,144 int64 or 12,288 int32 build tuples); beyond it the
-    size-class rule's two passes."""
+    table (6,144 int64 or 12,288 int32 build tuples); beyond it two passes:
+    9 + b2 bits with items of about a third of a 64 KiB table's build tuples
+    for probe sides of 16 |R| or more, else the size-class rule."""
     assert fij.plan_bits(dtype, one_pass_max, 1 << 28) == (9, 0)
     assert fij.plan_bits(dtype, 1000, 1 << 14) == (9, 0)
-    b1, b2 = fij.plan_bits(dtype, one_pass_max + 1, 1 << 28)
-    _, r1, r2 = fij.autotune(dtype, one_pass_max + 1, 1 << 28)
-    assert (b1, b2) == (r1, r2) and b2 > 0
+    per_item = 1024 if dtype == np.int64 else 2048
+    for n_r in (one_pass_max + 1, 4 * one_pass_max, 1 << 26):
+        b1, b2 = fij.plan_bits(dtype, n_r, 16 * n_r)
+        assert b1 == 9 and (n_r >> 9) / (1 << b2) <= per_item < (n_r >> 9) / (1 << (b2 - 1))
+        _, r1, r2 = fij.autotune(dtype, n_r, 8 * n_r)
+        assert fij.plan_bits(dtype, n_r, 8 * n_r) == (r1, r2) and r2 > 0
     with pytest.raises(FijError):
         fij.check(fij.lib.fij_join_plan_bits(7, 10, 10, C.byref(C.c_uint32()), C.byref(C.c_uint32())))
