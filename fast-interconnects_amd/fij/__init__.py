@@ -860,8 +860,9 @@ _proto("fij_join_plan_bits", C.c_int, C.c_int, _u64, _u64, C.POINTER(_u32), C.PO
 
 def plan_bits(dtype, build_len, probe_len):
     """(radix_bits_1, radix_bits_2) for one join run alone: a single pass of
-    9 bits when the build side's partitions fit block tables, else the
-    size-class rule's two passes."""
+    9 bits when the build side's partitions fit block tables; else two
+    passes, with items of about a third of a block table for probe sides of
+    16 |R| or more, else the size-class rule's."""
     b1, b2 = _u32(), _u32()
     check(lib.fij_join_plan_bits(int(_ktype(dtype)), build_len, probe_len, C.byref(b1), C.byref(b2)))
     return b1.value, b2.value

@@ -367,7 +367,9 @@ typedef struct fij_join_timing {
  * most 2/3 of a 128 KiB block-shared LDS table (|R| <= 2^21 at int64, 2^22
  * at int32; plans whose partitions also fit 64 KiB tables use those), so
  * that the probe side is partitioned once and its partitions are streamed
- * against them; otherwise the two-pass bits of fij_join_autotune.
+ * against them.  Otherwise two passes: for a probe side of 16 |R| or more,
+ * 9 + b2 bits with items of about a third of a 64 KiB block table's build
+ * tuples (1,024 int64, 2,048 int32), else the bits of fij_join_autotune. 
  * No counterpart in the reference, whose caller chooses the bits. */
 fij_status fij_join_plan_bits(fij_type type, uint64_t build_len, uint64_t probe_len, uint32_t *radix_bits_1,
                               uint32_t *radix_bits_2);
