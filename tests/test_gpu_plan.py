@@ -154,3 +154,24 @@ def test_plan_register_join_shapes(dtype, log2r, log2s):
         sp = orc.gen_iota(n_s, 3, dtype)
         got, _ = run_plan(rk, rp, sk, sp, 6, 5)
         assert got == orc.sort_merge_join(rk, rp, sk, sp), probe
+
+
+@pytest.mark.parametrize("skewed", [False, True])
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_single_pass_big_block_tables(dtype, skewed):
+    """Single-pass plans whose build partitions need the 128 KiB block tables
+    (fij_join_plan_bits: |R| of 1.5 x 2^20 int64 or 3 x 2^20 int32 at 9 bits).
+    Skewed: a quarter of the build keys fall into one partition, which no
+    block table holds (the wave-table path joins it)."""
+    n = (3 << 19) if dtype == np.int64 else (3 << 20)
+    assert fij.plan_bits(dtype, n, 4 * n) == (9, 0)
+    rk = orc.gen_perm(n, 21, dtype, 1)
+    if skewed:
+        rk[: n // 4] = (np.arange(n // 4, dtype=np.int64) * 512 + (1 << 30)).astype(dtype)  # all in partition 0
+    rp = orc.gen_uniform(n, 21, 1, -(1 << 20), 1 << 21, dtype)
+    sk = orc.gen_uniform(4 * n, 21, 2, 1, n + n // 4, dtype)
+    if skewed:
+        sk[: n // 2] = rk[: n // 2]
+    sp = orc.gen_iota(4 * n, 3, dtype)
+    got, _ = run_plan(rk, rp, sk, sp, 9, 0)
+    assert got == orc.sort_merge_join(rk, rp, sk, sp)
