@@ -568,7 +568,7 @@ def served_main(a):
 
 def ooc_main(a):
     """Out-of-core join: R = S = 2^log2n int64 tuples in pinned host memory
-    (the C2 generators), pass-1 partitions cached in device memory up to
+    (the plain join's generators), pass-1 partitions cached in device memory up to
     --cache-gb and spilled to pinned host memory beyond it, pass 2 and the
     join streamed over partition groups.  Pass 1 on the GPU (reading the
     relations over the host link) or, with --pass1-cpu, on host threads:
@@ -627,7 +627,7 @@ def ooc_main(a):
 
 def np_main(a):
     """The no-partitioning join (the hashjoin binary's comparison point):
-    R = S = 2^log2n int64 tuples with the C2 generators, a global
+    R = S = 2^log2n int64 tuples with the plain join's generators, a global
     linear-probing table of 2 * next_pow2(|R|) entries, grid 2 x CUs of 256
     threads (hashjoin/src/main.rs:383-390); and the same probe side against
     a build side of 2^22 tuples, whose table (128 MB) fits the GPU's last

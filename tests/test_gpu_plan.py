@@ -3,7 +3,8 @@ against the oracle: same inputs, identical count and wrapping sums.
 
 Mirrors radix-join/tests/test_gpu_radix_join.rs (closed form N(N+1)/2 for
 PK/FK joins, :146-149) and adds duplicate keys, misses, negative payloads,
-single-pass plans and the BASELINE shapes at reduced size.
+single-pass plans and the benchmark shapes (2^20 x 2^24 int32 in one pass of 8 bits,
+2^27 x 2^27 int64 in 9 + 9 bits at 1/16 size).
 """
 import numpy as np
 import pytest
@@ -46,7 +47,7 @@ def test_plan_duplicates_misses(dtype, bits):
     assert got == orc.radix_join(rk, rp, sk, sp, bits[0], bits[1], threads=4)
 
 
-def test_plan_c1_shape():
+def test_plan_2e20_x_2e24_i32_one_pass_8_bits():
     """CPU-runnable shape: R 2^20 permutation of 1..2^20 (payload = key), S 2^24
     uniform over R's keys (payload = row index), one pass of 8 bits."""
     n_r, n_s = 1 << 20, 1 << 24
@@ -59,8 +60,8 @@ def test_plan_c1_shape():
     assert got[0] == n_s
 
 
-def test_plan_c2_shape_reduced():
-    """Full-scale shape at 1/16 size: R = S = 2^23 i64, 9 + 9 bits."""
+def test_plan_2e23_i64_9_9_bits():
+    """The 2^27 x 2^27 int64 shape at 1/16 size: R = S = 2^23, 9 + 9 bits."""
     n = 1 << 23
     rk = orc.gen_perm(n, 42, np.int64, 1)
     rp = rk.copy()
