@@ -529,9 +529,19 @@ def served_main(a):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(stream)))
     wall, lat = run(arrivals)
     total = float(tuples[stream].sum())
-    # per request: the probe side's histogram, two passes and join read
-    # (k + 5t per tuple) plus the join's read of the resident build side (t)
-    b_alg = float(sum(int(n_p[j]) * 88 + int(n_b[probes[j][0]]) * 16 for j in stream))
+    # per request, by how its build side is resident: a hash table -- the
+    # probe side read once and one table entry per probe tuple (2t); one pass
+    # -- the probe side's histogram, scatter and join read (k + 3t) plus the
+    # build side's read (t); two passes -- k + 5t per probe tuple plus t per
+    # build tuple
+    def request_bytes(j):
+        nb, np_ = int(n_b[probes[j][0]]), int(n_p[j])
+        if a.table_build_max and nb <= a.table_build_max:
+            return np_ * 32
+        if fij.plan_bits(np.int64, nb, 1 << 28)[1] == 0:
+            return np_ * 56 + nb * 16
+        return np_ * 88 + nb * 16
+    b_alg = float(sum(request_bytes(j) for j in stream))
     plain_s = float(np.array(t_plain)[stream].sum())
     alloc_s = float(np.array(t_alloc)[stream].sum())
     value = total / wall
