@@ -187,7 +187,9 @@ fij_status grow_arena(ClassArena& a, KeyType kt, uint32_t b1, uint32_t b2, uint6
     if ((e = arena_alloc(n, &ps.off1, parts)) != hipSuccess || (e = arena_alloc(n, &ps.plen, parts)) != hipSuccess ||
         (e = arena_alloc(n, &ps.pstart, parts + 1)) != hipSuccess ||
         (e = arena_alloc(n, &ps.tile_start, parts + 1)) != hipSuccess ||
-        (e = arena_alloc(n, &r1, n.rel1[s] * tb)) != hipSuccess || (e = arena_alloc(n, &r2, n.rel2[s] * tb)) != hipSuccess ||
+        // one spare tuple after rel1 (pass 2 reads 8-byte tuples in aligned pairs)
+        (e = arena_alloc(n, &r1, (n.rel1[s] + 1) * tb)) != hipSuccess ||
+        (e = arena_alloc(n, &r2, n.rel2[s] * tb)) != hipSuccess ||
         (e = arena_alloc(n, &ps.meta, (n.tiles[s] + 1) * F2)) != hipSuccess)
       break;
     ps.rel1 = r1;
@@ -372,7 +374,7 @@ fij_join_tuning class_rule(KeyType kt, uint64_t build_len, uint64_t probe_len) {
   // join's registers (at most 640 per side), split evenly over the passes
   // (pass 1 at most 9 bits: its LDS line buffers), and enough pass-1
   // partitions that a pair's probe side has at most 64 pass-2 tiles of 8192
-  // tuples (int32 tiles hold 16,384: at most 32 of them).
+  // tuples (int32 tiles hold 16,382: at most 33 of them).
   uint32_t b = std::max<uint32_t>(
       2, std::max(ceil_log2((build_len + 511) / 512), ceil_log2((probe_len + 511) / 512)));
   uint32_t b1 = std::min<uint32_t>(9, (b + 1) / 2);

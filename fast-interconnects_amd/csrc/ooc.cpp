@@ -378,7 +378,8 @@ extern "C" fij_status fij_ooc_join(const fij_ooc_config* cfg, const void* build_
     R.cut = P < F1 ? R.start(P) : R.padded_len;
     R.spill = r_host + R.cut * tb;  // stands for tuple index R.cut
     if (R.cut) {
-      OOC_HIP(A.dalloc(&R.cache, R.cut * tb));
+      // one spare tuple: pass 2 reads 8-byte tuples in aligned pairs
+      OOC_HIP(A.dalloc(&R.cache, (R.cut + 1) * tb));
       OOC_HIP(hipMemcpyAsync(R.cache, r_host, R.cut * tb, hipMemcpyHostToDevice, main_st));
       OOC_HIP(hipEventRecord(ev[3], main_st));
       for (int s = 0; s < NS; ++s) OOC_HIP(hipStreamWaitEvent(st[s], ev[3], 0));
@@ -488,7 +489,7 @@ extern "C" fij_status fij_ooc_join(const fij_ooc_config* cfg, const void* build_
       while (P < F1 && s.end(P) <= cache_tuples[r]) ++P;
       s.cut_part = P;
       s.cut = P < F1 ? s.start(P) : s.padded_len;
-      if (s.cut) OOC_HIP(A.dalloc(&s.cache, s.cut * tb));
+      if (s.cut) OOC_HIP(A.dalloc(&s.cache, (s.cut + 1) * tb));  // + the spare tuple of pass 2
       if (s.cut < s.padded_len) {
         // the spill region keeps the relation's alignment modulo 128 bytes,
         // so the scatter's full lines stay aligned there too

@@ -189,12 +189,14 @@ __device__ __forceinline__ U uniform_load(const U* ptr) {
 template <typename K>
 constexpr uint32_t kBlockTableSlots = 65536 / (2 * sizeof(K));
 
-// Tuples per pass-2 tile: 128 KiB of LDS (16,384 8-byte or 8,192 16-byte
-// tuples).  A tile holds one run per pass-2 digit, and the join's gathers
-// of a run cost by the cache lines it touches: longer runs, fewer partial
-// lines.
+// Tuples per pass-2 tile: 128 KiB of LDS (8,192 16-byte tuples; 16,382
+// 8-byte tuples: their pairs are loaded from the 16-byte aligned tuple at
+// or before the tile's start, and 8,192 pairs must cover the tile from
+// either alignment).  A tile holds one run per pass-2 digit, and the join's
+// gathers of a run cost by the cache lines it touches: longer runs, fewer
+// partial lines.
 template <typename K>
-constexpr uint32_t kTile2 = 131072 / sizeof(Tuple<K>);
+constexpr uint32_t kTile2 = sizeof(K) == 4 ? 131072 / sizeof(Tuple<K>) - 2 : 131072 / sizeof(Tuple<K>);
 
 // Raw buffer resource over `bytes` bytes at `base` (uniform): loads past the
 // end return zeros.  Third-dword flags of a raw 32-bit buffer on gfx9-family
@@ -203,16 +205,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void* base, ui
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
 }
 
-// Non-temporal 8-byte tuple load from a buffer resource (cache policy bit 1 = nt).
-template <typename K>
-__device__ __forceinline__ Tuple<K> buffer_load_tuple_nt(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
-  static_assert(sizeof(K) == 4, "8-byte tuples");
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, static_cast<int>(voff), static_cast<int>(soff), 2);
-  Tuple<K> t;
-  t.key = static_cast<K>(v[0]);
-  t.value = static_cast<K>(v[1]);
-  return t;
-}
 
 // One pass-2 tile: relation, pass-1 partition p, tile j of p, its length and
 // its first tuple in rel1.
@@ -220,6 +212,7 @@ template <typename K>
 struct TileRef {
   uint32_t rel, p, j, n;
   const Tuple<K>* src;
+  uint32_t a;  // 8-byte tuples: 1 when src is not 16-byte aligned
 };
 
 // Pass 2 of both relations (the sides in d.sides) as one persistent launch:
@@ -231,7 +224,7 @@ struct TileRef {
 // current tile is ranked, sorted and written.
 template <typename K, uint32_t TILE2, uint32_t BLOCK>
 __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
-  constexpr uint32_t ITEMS = TILE2 / BLOCK;
+  constexpr uint32_t ITEMS = (TILE2 + BLOCK - 1) / BLOCK;  // tuple slots per thread
   using T = Tuple<K>;
   extern __shared__ uint64_t lds_u64[];
   T* sorted = reinterpret_cast<T*>(lds_u64);                    // [TILE2]
@@ -258,7 +251,23 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     r.j = u - uniform_load(s.tile_start + lo);
     r.n = min(TILE2, uniform_load(s.plen + lo) - r.j * TILE2);
     r.src = static_cast<const T*>(s.rel1) + uniform_load(s.off1 + lo) + static_cast<uint64_t>(r.j) * TILE2;
+    r.a = sizeof(K) == 4 ? static_cast<uint32_t>((reinterpret_cast<uintptr_t>(r.src) / 8) & 1u) : 0u;
     return r;
+  };
+  // Tile index of a thread's tuple slot k.  8-byte tuples are loaded in
+  // pairs from the 16-byte aligned tuple at or before the tile's start (a
+  // window that starts a tuples early): slot k holds window position
+  // 2 * ((k / 2) * BLOCK + tid) + k % 2.  16-byte tuples: k * BLOCK + tid.
+  auto slot_pos = [&](uint32_t k) -> uint32_t {
+    if constexpr (sizeof(K) == 4) return 2 * ((k / 2) * BLOCK + tid) + (k & 1u);
+    return k * BLOCK + tid;
+  };
+  // Whether slot k holds a tuple of the tile (window positions a .. a + n - 1
+  // for 8-byte tuples)
+  auto slot_live = [&](const TileRef<K>& r, uint32_t k) -> bool {
+    const uint32_t w = slot_pos(k);
+    if constexpr (sizeof(K) == 4) return w >= r.a && w - r.a < r.n;
+    return w < r.n;
   };
   // Non-temporal loads.  8-byte tuples (16 per thread) use buffer loads over
   // the tile: one offset register for all of them (the step goes in the
@@ -268,11 +277,21 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
   // such lanes count nothing.
   auto load = [&](const TileRef<K>& r, T (&v)[ITEMS]) {
     if constexpr (sizeof(K) == 4) {
-      const __amdgpu_buffer_rsrc_t rs = tile_rsrc(r.src, r.n * static_cast<uint32_t>(sizeof(T)));
+      // the pairs that hold the tile's tuples (a pair past them reads zeros).
+      // The first pair may start one tuple before the tile (padding in front
+      // of a partition, or the previous tile) and the last end one tuple
+      // after it (the next tile, the next partition's padding, or the one
+      // spare tuple every pass-1 buffer has after its end).
+      const __amdgpu_buffer_rsrc_t rs = tile_rsrc(r.src - r.a, ((r.n + r.a + 1) / 2) * 16u);
 #pragma unroll
-      for (uint32_t k = 0; k < ITEMS; ++k)
-        v[k] = buffer_load_tuple_nt<K>(rs, tid * static_cast<uint32_t>(sizeof(T)),
-                                       k * BLOCK * static_cast<uint32_t>(sizeof(T)));
+      for (uint32_t k = 0; k < ITEMS; k += 2) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs, static_cast<int>(tid * 16u),
+                                                             static_cast<int>((k / 2) * BLOCK * 16u), 2);
+        v[k].key = static_cast<K>(x[0]);
+        v[k].value = static_cast<K>(x[1]);
+        v[k + 1].key = static_cast<K>(x[2]);
+        v[k + 1].value = static_cast<K>(x[3]);
+      }
     } else {
 #pragma unroll
       for (uint32_t k = 0; k < ITEMS; ++k) v[k] = load_tuple_nt<K>(r.src + min(k * BLOCK + tid, r.n - 1));
@@ -284,7 +303,7 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     uint32_t dr[ITEMS];
 #pragma unroll
     for (uint32_t k = 0; k < ITEMS; ++k) {
-      const bool act = k * BLOCK + tid < r.n;
+      const bool act = slot_live(r, k);
       const uint32_t dig = key_to_partition<K>(v[k].key, mask2, d.bits1);
       // 8-byte tuples: one LDS atomic per tuple is faster; 16-byte tuples
       // (half as many per tile) gain from aggregating lanes of a digit
@@ -302,7 +321,7 @@ __global__ __launch_bounds__(BLOCK) void sort2_kernel(PipelineDevice d) {
     for (uint32_t i = tid; i < F2; i += BLOCK) meta[i] = cnt[i] | ((i + 1 < F2 ? cnt[i + 1] : r.n) << 16);
 #pragma unroll
     for (uint32_t k = 0; k < ITEMS; ++k)
-      if (k * BLOCK + tid < r.n) sorted[cnt[dr[k] & 0xffffu] + (dr[k] >> 16)] = v[k];
+      if (slot_live(r, k)) sorted[cnt[dr[k] & 0xffffu] + (dr[k] >> 16)] = v[k];
     __syncthreads();
     T* out = static_cast<T*>(s.rel2) + uniform_load(s.pstart + r.p) + static_cast<uint64_t>(r.j) * TILE2;
     if constexpr (sizeof(K) == 4) {

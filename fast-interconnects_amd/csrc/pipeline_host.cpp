@@ -35,7 +35,9 @@ hipError_t alloc_pipeline_side(PipelineSide& s, KeyType t, uint64_t n, uint32_t 
   if ((e = alloc_n(&s.counts1, static_cast<uint64_t>(F1) * G1, allocs)) != hipSuccess) return e;
   if ((e = alloc_n(&s.scan1, s.scan1_len, allocs)) != hipSuccess) return e;
   uint8_t* rel1 = nullptr;
-  if ((e = alloc_n(&rel1, s.padded_len1 * tb, allocs)) != hipSuccess) return e;
+  // one spare tuple after the end: pass 2 reads 8-byte tuples in aligned
+  // pairs, and the last pair may end one tuple past the relation
+  if ((e = alloc_n(&rel1, (s.padded_len1 + 1) * tb, allocs)) != hipSuccess) return e;
   s.rel1 = rel1;
   if ((e = alloc_n(&s.plen, F1, allocs)) != hipSuccess) return e;
   if ((e = alloc_n(&s.pstart, F1 + 1, allocs)) != hipSuccess) return e;
