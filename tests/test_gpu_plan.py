@@ -89,6 +89,30 @@ def test_plan_repeat_and_chunks():
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("bits", [(9, 9), (6, 0)])
+def test_plan_empty_sides(dtype, bits):
+    """An empty build side, an empty probe side and both (zero-length views
+    of real buffers) through a plan of 2^16 x 2^18 tuples join nothing; the
+    full join through the same plan afterwards equals the oracle's, so the
+    empty joins left no state behind."""
+    n_r, n_s = 1 << 16, 1 << 18
+    rk = orc.gen_perm(n_r, 9, dtype, 1)
+    rp = orc.gen_uniform(n_r, 9, 2, -(1 << 20), 1 << 21, dtype)
+    sk = orc.gen_uniform(n_s, 9, 3, 1, n_r + n_r // 4, dtype)
+    sp = orc.gen_iota(n_s, 0, dtype)
+    d = [to_dev(a) for a in (rk, rp, sk, sp)]
+    e = [a.view(0, 0) for a in d]
+    plan = fij.JoinPlan(dtype, n_r, n_s, *bits)
+    res = fij.DeviceArray(3, np.uint64)
+    for cols in ((e[0], e[1], d[2], d[3]), (d[0], d[1], e[2], e[3]), tuple(e)):
+        plan.enqueue_sized(*cols, res)
+        assert tuple(int(x) for x in res.to_host()) == (0, 0, 0)
+    plan.enqueue_sized(*d, res)
+    assert tuple(int(x) for x in res.to_host()) == orc.sort_merge_join(rk, rp, sk, sp)
+    assert fij.device_status() == 0
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
 def test_null_keys_never_join(dtype):
     rk = orc.gen_uniform(200_000, 4, 1, -3, 50_000, dtype)
     sk = orc.gen_uniform(600_000, 4, 2, -3, 50_000, dtype)
