@@ -399,6 +399,23 @@ def batched_main(a):
     print(json.dumps(out))
 
 
+def latency_by_slot_class(lat, probe_len, slot_classes, alone_s):
+    """Latency percentiles of the requests grouped by the smallest slot class
+    that holds their probe side (the server may run one in a larger free
+    slot), beside the median time of the same requests joined alone by the
+    plain launch: what a request waits for beyond its own work."""
+    out = {}
+    lo = 0
+    for _, cap in slot_classes:
+        sel = (probe_len > lo) & (probe_len <= cap)
+        if sel.any():
+            out[f"probe <= {cap}"] = {"requests": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                                      "p99_ms": float(np.percentile(lat[sel], 99)),
+                                      "alone_p50_ms": float(np.median(alone_s[sel]) * 1000.0)}
+        lo = cap
+    return out
+
+
 def served_main(a):
     """Served joins: a Poisson stream of requests against a pool of a.builds
     int64 build sides (|R| log-uniform 2^12..2^26, keys a permutation of
@@ -562,6 +579,8 @@ def served_main(a):
         "offered_rate_per_s": rate, "capacity_per_s": capacity,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
+        "latency_by_slot_class": latency_by_slot_class(lat, n_p[stream], slot_classes,
+                                                       np.array(t_plain)[stream]),
         "saturated": {"tuples_per_s": total / wall_sat, "p50_ms": float(np.percentile(lat_sat, 50)),
                       "p99_ms": float(np.percentile(lat_sat, 99))},
         "plain_per_join": {"seconds": plain_s, "tuples_per_s": total / plain_s,
